@@ -1,0 +1,201 @@
+"""Benchmark of the local ensemble square-root analysis on one MI355X.
+
+One step = one call of the analysis (oak_assim's ensemble path: H·E, anomalies,
+selection, per-zone S^T R^-1 S, eigensolve, transform, Ea) over the whole
+synthetic ensemble, with every input already resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--config 1] [--precision fp64]
+
+Prints one JSON line (rank 0).  The metric is analysed state elements per
+second: n / t_step, an element being one entry of xa with its N-member row of
+Ea.  ``roofline`` is computed for the dominant kernel from device-event times
+recorded inside the timed steps; ``cpu_baseline`` times the oracle (OAK's
+algorithm restated in C, LAPACK dsyev + BLAS dgemm, OpenMP over zones) on a
+contiguous sample of zones of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12   # MI355X FP64 (vector and matrix), FLOP/s
+PEAK_HBM = 8.0e12     # MI355X HBM3E, B/s
+
+
+def build_problem(cfg, device):
+    from oak_amd import synth
+    if cfg == 0:
+        import torch
+        P = synth.config0()
+        E = torch.as_tensor(np.ascontiguousarray(P.E.T)).to(device).t()
+        return synth.DeviceProblem(P, E), "64x64 grid, N=16, 400 obs"
+    if cfg == 1:
+        return synth.config1_gpu(device), "360x180x30x4+SSH (n=7,840,800), N=64, 201k obs"
+    if cfg == 2:
+        return synth.config2_gpu(device=device), "360x180x30x4+SSH (n=7,840,800), N=128, clustered obs"
+    raise SystemExit("unknown config %d" % cfg)
+
+
+def device_inputs(DP, device):
+    import torch
+    from oak_amd.assimilation import Geometry, coo_to_csr
+    P = DP.host
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(device, dt)
+    g = Geometry(t(P.zone_ptr, torch.int64), t(P.zpos), t(P.corr_len), t(P.max_len), t(P.ox),
+                 t(P.oy), metric=P.metric, loctype=P.loctype)
+    rp, co, va = coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    H = (t(rp, torch.int64), t(co, torch.int32), t(va))
+    return g, H, t(P.Hshift), t(P.yo), t(P.rmse2)
+
+
+def flops_model(N, counts, zone_sizes):
+    """Algorithmic FLOP per analysis, per phase, from the per-zone local
+    observation counts p_z (zones with p_z > 0):
+      gram   2 N^2 p + 2 N p          (S^T R^-1 S and S^T R^-1 d)
+      eig    9 N^3                    (symmetric eigensolve with vectors)
+      trans  6 N^3                    (Omega, U^T Omega, U (.))
+      update 2 s N^2                  (Sf_z T)"""
+    act = counts > 0
+    p = counts[act].astype(np.float64)
+    s = zone_sizes[act].astype(np.float64)
+    return {
+        "gram": float(np.sum(2.0 * N * N * p + 2.0 * N * p)),
+        "eig": float(9.0 * N ** 3 * act.sum()),
+        "trans": float(6.0 * N ** 3 * act.sum()),
+        "update": float(np.sum(2.0 * s * N * N)),
+    }
+
+
+def cpu_baseline(DP, target_s=15.0):
+    """Oracle (LAPACK backend, OpenMP dynamic over zones, OAK's brute-force
+    selection) on a contiguous slice of zones in the middle of the domain."""
+    import torch
+    from oracle import oracle as O
+    P = DP.host
+    E = DP.E
+    N = E.shape[1]
+    lapack = O.load_lapack()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    # obs-space anomalies for every observation (computed on the GPU: this
+    # leg times the zone loop of locAnalysis, not H·E)
+    rows = torch.as_tensor(P.Hi.astype(np.int64), device=E.device)
+    cols = torch.as_tensor(P.Hj.astype(np.int64), device=E.device)
+    vals = torch.as_tensor(P.Hs, device=E.device)
+    HE = torch.zeros((P.m, N), dtype=torch.float64, device=E.device)
+    HE.index_add_(0, rows, vals[:, None] * E[cols, :].double())
+    Hxf = HE.mean(dim=1)
+    HSf = ((HE - Hxf[:, None]) / np.sqrt(N - 1)).cpu().numpy()
+    Hxf = Hxf.cpu().numpy()
+    nz = P.nzones
+    zc = nz // 2
+
+    def run(nzs):
+        z0, z1 = zc, min(nz, zc + nzs)
+        i0, i1 = int(P.zone_ptr[z0]), int(P.zone_ptr[z1])
+        Ez = E[i0:i1, :].double().cpu().numpy()
+        xf = Ez.mean(axis=1)
+        Sf = np.asfortranarray((Ez - xf[:, None]) / np.sqrt(N - 1))
+        geo = dict(zone_ptr=P.zone_ptr[z0:z1 + 1] - i0, zpos=P.zpos[z0:z1], L=P.corr_len[z0:z1],
+                   cutoff=P.max_len[z0:z1], metric=P.metric, loctype=P.loctype, ox=P.ox, oy=P.oy)
+        t0 = time.perf_counter()
+        O.loc_analysis_geo(geo, xf, Hxf, P.yo, P.rmse2, Sf, HSf, nthreads=cores)
+        return time.perf_counter() - t0, i1 - i0, z1 - z0
+
+    t, ne, nzs = run(max(cores, 16))
+    per_zone = t / nzs
+    print("# cpu baseline probe: %d zones in %.2f s" % (nzs, t), file=sys.stderr, flush=True)
+    nzs2 = int(min(nz - zc, max(cores, target_s / max(per_zone, 1e-6))))
+    t, ne, nzs = run(nzs2)
+    return {"value": ne / t, "unit": "analysed state elems/s", "cores": cores, "kind": "port",
+            "sample": "%d contiguous zones (%d state elements) of the same workload, zone loop "
+                      "of locAnalysis with OAK's brute-force selection; %s eigensolver, %.1f s"
+                      % (nzs, ne, "LAPACK dsyev (OpenBLAS)" if lapack else "Jacobi", t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from oak_amd import Context, OAKG_FP64, OAKG_MIXED
+    from oak_amd.assimilation import ensemble_analysis
+    device = torch.device("cuda:0")
+    t0 = time.time()
+    DP, desc = build_problem(args.config, device)
+    P = DP.host
+    g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
+    gen_s = time.time() - t0
+    print("# generated %s in %.1f s" % (desc, gen_s), file=sys.stderr, flush=True)
+    prec = OAKG_MIXED if args.precision == "mixed" else OAKG_FP64
+    ctx = Context(0, prec)
+    E = DP.E
+    n, N = E.shape
+    Ea = torch.empty((N, n), dtype=E.dtype, device=device).t()
+    ctx.set_timing(True)
+    for i in range(args.warmup):
+        ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
+        torch.cuda.synchronize()
+        print("# warmup %d done: %s" % (i, ctx.stats()), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    phase = {"gram": 0.0, "eig": 0.0, "update": 0.0, "select": 0.0, "prep": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
+        st = ctx.stats()  # phase times of this call (device events)
+        for k in phase:
+            phase[k] += st["ms_" + k]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    for k in phase:
+        phase[k] /= args.steps
+    counts = ctx.counts(P.nzones)
+    zs = np.diff(P.zone_ptr)
+    F = flops_model(N, counts, zs)
+    # dominant kernel
+    kern = max(("gram", "eig"), key=lambda k: phase[k])
+    if kern == "gram":
+        ach = F["gram"] / (phase["gram"] * 1e-3)
+        kname = "k_gram (S^T R^-1 S, v_mfma_f64_16x16x4f64)"
+    else:
+        ach = (F["eig"] + F["trans"]) / (phase["eig"] * 1e-3)
+        kname = "k_eig+k_tmat (one-sided Jacobi, FP64)"
+    roof = {"bound": "mfma", "kernel": kname, "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None}
+    out = {
+        "metric": "analysed state elems/s",
+        "value": n / (ms * 1e-3),
+        "unit": "analysed state elems/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == OAKG_FP64 else "f32/bf16+f64",
+        "data": "synthetic (seeded GRF ensembles and observations)",
+        "config": {"workload": "config%d: %s, %s" % (args.config, desc, args.precision),
+                   "n": int(n), "N": int(N), "m": int(P.m), "zones": int(P.nzones),
+                   "mean_p": float(counts.mean()), "max_p": int(counts.max()),
+                   "phase_ms": {k: round(v, 3) for k, v in phase.items()}},
+        "roofline": roof,
+    }
+    print("# timed %d steps: %.3f ms/step" % (args.steps, ms), file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(DP)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

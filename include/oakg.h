@@ -1,0 +1,184 @@
+/*
+ * oakg.h -- C ABI of the MI355X (gfx950) local ensemble square-root analysis.
+ *
+ * The entry points replace, for OAK's local scheme (schemetype = 1), the
+ * analysis interfaces below; every array is column-major (Fortran layout) with
+ * an explicit leading dimension, indices are 0-based, and state vectors are in
+ * OAK's zone-permuted order (each zone contiguous, assimilation.F90:869-880).
+ *
+ *   oakg_local_analysis      <- locAnalysis(zoneSize, selectObservations, xf, Hxf,
+ *                               yo, Sf, HSf, R, xa, Sa, amplitudes, localise_obs)
+ *                               rrsqrt.F90:433-466 (zone loop :258-426, per-zone
+ *                               solve analysisIncrement :100-190), with the
+ *                               selection of assimilation.F90:3683-3771 done on
+ *                               the GPU and R = DiagCovar(rmse^2)
+ *                               (covariance.F90:70-79, assimilation.F90:2120-2127)
+ *   oakg_local_analysis_csr  <- the same call with an arbitrary selectObservations
+ *                               callback, evaluated by the caller per zone
+ *                               (e.g. test/test_rrsqrt.F90:254-271)
+ *   oakg_ensemble_analysis   <- oak_assim(ntime, n, r, Ef, Ea)
+ *                               assimilation.F90:4919-4927, i.e. Assim's ensemble
+ *                               path :3094-3135, :3299-3364, :3558-3563 (H·E,
+ *                               anomalies, local analysis, inflation,
+ *                               maxCorrection, ensemble rebuild)
+ *   oakg_ensemble_analysis_f32  mixed-precision variant of the above (FP32
+ *                               ensembles, FP64 eigensolve); no OAK counterpart
+ *                               beyond oak_assim built with PRECISION=single
+ *   oakg_coo_to_csr          <- SparseMatrix (matoper.F90:30-39) as used by
+ *                               obsoper (assimilation.F90:2836-2898)
+ *   oakg_partition           <- initPartition (assimilation.F90:578-641) after
+ *                               the gap relabelling of :408-412
+ *   oakg_create/oakg_destroy <- oak_init/oak_done (assimilation.F90:4911-4933)
+ *                               for the device context only (no init file)
+ *
+ * Conventions: every function returns OAKG_OK (0) or an error code; the text
+ * of the last error of a context is returned by oakg_last_error.  With host
+ * pointers, outputs are written only on success.  A context is used by one
+ * host thread at a time; different contexts are independent.  Calls block
+ * until the results are in the output arrays.
+ */
+#ifndef OAKG_H
+#define OAKG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OAKG_OK 0
+#define OAKG_EARG 1   /* invalid argument (sizes, NULL arrays, rmse <= 0)           */
+#define OAKG_ENAN 2   /* NaN in the amplitudes: rrsqrt.F90:145-149 (ERROR_STOP)     */
+#define OAKG_EHIP 3   /* HIP runtime error                                          */
+#define OAKG_ENOMEM 4 /* device allocation failed                                   */
+#define OAKG_EUNSUP 5 /* configuration not supported (e.g. N above the maximum)     */
+
+#define OAKG_FP64 0  /* plain FP64 path                                             */
+#define OAKG_MIXED 1 /* FP32/BF16 contractions, FP64 eigensolve                     */
+
+#define OAKG_DEVICE_PTRS 1u /* flag: every array argument is a device pointer        */
+
+#define OAKG_CARTESIAN 0        /* metrictype, assimilation.F90:108-112 */
+#define OAKG_SPHERICAL 1
+#define OAKG_SPHERICAL_APPROX 2
+
+#define OAKG_MAX_N 128 /* largest ensemble size of this build */
+
+typedef struct oakg_ctx oakg_ctx;
+
+/* Zone layout and localisation geometry (assimilation.F90:210-229, 400-451). */
+typedef struct {
+  int64_t nzones;
+  const int64_t* zone_ptr; /* [nzones+1] start of each zone in the permuted state   */
+  const double* zpos;      /* [nzones*4] x,y,z,t of each zone's FIRST element,       */
+                           /*   one row of 4 per zone (assimilation.F90:3713-3740)  */
+  const double* corr_len;  /* [nzones] hCorrLengthToObs of that element  (L)        */
+  const double* max_len;   /* [nzones] hMaxCorrLengthToObs of that element (cutoff) */
+  int32_t metric;          /* OAKG_CARTESIAN / OAKG_SPHERICAL / ..._APPROX          */
+  int32_t loctype;         /* 1 horizontal distance, 2 |dz|, 3 |dt|                 */
+} oakg_zones;
+
+/* Observations (obsGridX/Y/Z/T, yo and the diagonal of R). */
+typedef struct {
+  int64_t m;
+  const double* x; /* [m] obsGridX (longitude or x)                  */
+  const double* y; /* [m] obsGridY (latitude or y)                   */
+  const double* z; /* [m] obsGridZ, may be NULL unless loctype == 2  */
+  const double* t; /* [m] obsGridT, may be NULL unless loctype == 3  */
+  const double* yo;       /* [m] observations                                     */
+  const double* rmse2;    /* [m] R = diag(rmse^2), every entry > 0                */
+  const uint8_t* excluded; /* [m] 1 = excluded (assimilation.F90:3086-3092), or NULL */
+} oakg_obs;
+
+/* Observation operator H (m x n), compressed by observation row. */
+typedef struct {
+  int64_t m, n, nnz;
+  const int64_t* row_ptr; /* [m+1]                                   */
+  const int32_t* col;     /* [nnz] column = permuted state index     */
+  const double* val;      /* [nnz]                                   */
+} oakg_csr;
+
+/* Per-call statistics (filled by every analysis call). */
+typedef struct {
+  int64_t zones;           /* zones in the call                                  */
+  int64_t zones_analysed;  /* zones with at least one relevant observation       */
+  int64_t obs_selected;    /* sum over zones of the local observation count p_z  */
+  int64_t pmax;            /* largest p_z                                        */
+  int32_t sweeps_max;      /* most Jacobi sweeps used by one zone                */
+  int32_t reserved;
+  double ms_total;         /* device time of the whole call                      */
+  double ms_prep, ms_select, ms_gram, ms_eig, ms_update; /* per phase (timing on) */
+} oakg_stats;
+
+int oakg_version(void);
+int oakg_create(oakg_ctx** ctx, int device, int precision);
+int oakg_destroy(oakg_ctx* ctx);
+const char* oakg_last_error(const oakg_ctx* ctx);
+/* Launch on this HIP stream (hipStream_t) instead of the default stream. */
+int oakg_set_stream(oakg_ctx* ctx, void* stream);
+/* 1 = record per-phase device times into oakg_stats (adds events). */
+int oakg_set_timing(oakg_ctx* ctx, int on);
+int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
+/* Per-zone local observation counts p_z of the last call (host array). */
+int oakg_get_counts(oakg_ctx* ctx, int64_t* counts, int64_t nzones);
+
+/* locAnalysis: xa = xf + Sf·a_z per zone, Sa = Sf·T_z (rrsqrt.F90:100-190,
+ * 258-466).  xf, xa: [n]; Sf, Sa: n x N; Hxf: [m]; HSf: m x N.  amplitudes
+ * (N x nzones, may be NULL) is filled only when localise_obs == 0, as in
+ * rrsqrt.F90:378-383.  localise_obs = 1 is OAK's default. */
+int oakg_local_analysis(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs, int32_t N,
+                        const double* xf, const double* Hxf, const double* Sf, int64_t ldSf,
+                        const double* HSf, int64_t ldHSf, double* xa, double* Sa, int64_t ldSa,
+                        double* amplitudes, int32_t localise_obs, uint32_t flags);
+
+/* Same, with the selection supplied per zone as CSR: zone z uses observations
+ * sel_idx[sel_ptr[z] .. sel_ptr[z+1]) with taper weights sel_w (the weight
+ * array that selectObservations returns).  With localise_obs == 0 the list of
+ * a zone must hold all m observations (or none). */
+int oakg_local_analysis_csr(oakg_ctx* ctx, int64_t nzones, const int64_t* zone_ptr,
+                            const int64_t* sel_ptr, const int32_t* sel_idx, const double* sel_w,
+                            const oakg_obs* obs, int32_t N, const double* xf, const double* Hxf,
+                            const double* Sf, int64_t ldSf, const double* HSf, int64_t ldHSf,
+                            double* xa, double* Sa, int64_t ldSa, double* amplitudes,
+                            int32_t localise_obs, uint32_t flags);
+
+/* The selection alone (assimilation.F90:3683-3771) for every zone: writes
+ * sel_ptr[nzones+1] and, if sel_idx/sel_w are not NULL, up to cap entries of
+ * observation indices and taper weights (zone by zone, in an order of the
+ * implementation's choosing).  *total receives the number of entries. */
+int oakg_select_observations(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs,
+                             int32_t localise_obs, int64_t* sel_ptr, int32_t* sel_idx,
+                             double* sel_w, int64_t cap, int64_t* total, uint32_t flags);
+
+/* oak_assim: Ef (n x N ensemble, zone-permuted) -> Ea.  H·Ef + Hshift gives the
+ * observed ensemble; anomalies are scaled by 1/sqrt(N-1) (ppdef.h:52);
+ * Sa is multiplied by inflation (inflation.mult) and xa - xf is clamped to
+ * +-max_correction (Model.maxCorrection; pass HUGE_VAL for none). */
+int oakg_ensemble_analysis(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs,
+                           const oakg_csr* H, const double* Hshift, int32_t N,
+                           const double* Ef, int64_t ldEf, double* Ea, int64_t ldEa,
+                           double inflation, double max_correction, uint32_t flags);
+
+/* Mixed-precision ensemble analysis: FP32 ensembles, the contractions
+ * S = H·A', S^T R^-1 S and A'·W in FP32/BF16 with FP32 accumulation, the
+ * N x N eigensolve, (1+lambda)^-1/2 and the R^-1 taper scaling in FP64.
+ * Requires a context created with OAKG_MIXED. */
+int oakg_ensemble_analysis_f32(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs,
+                               const oakg_csr* H, const float* Hshift, int32_t N,
+                               const float* Ef, int64_t ldEf, float* Ea, int64_t ldEa,
+                               double inflation, double max_correction, uint32_t flags);
+
+/* Host helpers ------------------------------------------------------------ */
+/* Stable COO -> CSR by row (entries of a row keep their COO order). */
+int oakg_coo_to_csr(int64_t m, int64_t nnz, const int32_t* row, const int32_t* col,
+                    const double* val, int64_t* row_ptr, int32_t* col_out, double* val_out);
+/* Zone layout from a partition vector: returns nzones (or -1), fills
+ * zone_size[nzones], zone_index[n] (permuted position -> original index,
+ * stable) and inv_zone_index[n]. */
+int64_t oakg_partition(int64_t n, const int64_t* partition, int64_t* zone_size,
+                       int64_t* zone_index, int64_t* inv_zone_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OAKG_H */

@@ -1,0 +1,235 @@
+"""Host-side mirror of the parts of OAK's assimilation module on the local
+analysis path; every computation runs on the GPU through the C ABI.
+
+* ``Geometry``: the zone layout and localisation inputs that OAK keeps as
+  module state (zoneSize/startIndexZones, the first element's coordinates,
+  hCorrLengthToObs/hMaxCorrLengthToObs, metrictype, loctype;
+  assimilation.F90:210-229, 400-451, 3683-3740) plus obsGridX/Y/Z/T.
+* ``loc_analysis``: locAnalysis with OAK's own selectObservations
+  (assimilation.F90:3683-3771) evaluated on the GPU.
+* ``ensemble_analysis``: Assim's ensemble path for the local scheme, i.e.
+  oak_assim(ntime, n, r, Ef, Ea) (assimilation.F90:4919-4927, 3094-3364).
+* ``select_observations``: the selection alone, for inspection.
+* ``coo_to_csr`` / ``partition``: SparseMatrix -> CSR and initPartition.
+
+Arrays may be numpy (host; copied to and from the GPU) or torch tensors on
+the GPU (used in place).  Device matrices are column-major: a torch tensor of
+shape (rows, cols) with stride (1, ld).
+"""
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import Csr, Obs, Zones
+from .rrsqrt import default_context
+
+CARTESIAN, SPHERICAL, SPHERICAL_APPROX = 0, 1, 2
+
+
+def _is_dev(a):
+    return a is not None and not isinstance(a, np.ndarray) and hasattr(a, "data_ptr")
+
+
+@dataclass
+class Geometry:
+    zone_ptr: object          # [nz+1] int64
+    zpos: object              # [nz, 4] float64: x, y, z, t of each zone's first element
+    corr_len: object          # [nz] float64 (L)
+    max_len: object           # [nz] float64 (cutoff)
+    ox: object                # [m] obsGridX
+    oy: object                # [m] obsGridY
+    oz: Optional[object] = None
+    ot: Optional[object] = None
+    metric: int = CARTESIAN
+    loctype: int = 1
+
+    @property
+    def nzones(self):
+        return len(self.zone_ptr) - 1
+
+    def device(self):
+        return _is_dev(self.zone_ptr)
+
+    def host_arrays(self):
+        """Returns a copy with numpy arrays of the right dtypes and layouts."""
+        f = lambda a, t: None if a is None else np.ascontiguousarray(a, dtype=t)
+        return Geometry(f(self.zone_ptr, np.int64), f(self.zpos, np.float64),
+                        f(self.corr_len, np.float64), f(self.max_len, np.float64),
+                        f(self.ox, np.float64), f(self.oy, np.float64), f(self.oz, np.float64),
+                        f(self.ot, np.float64), self.metric, self.loctype)
+
+    def structs(self, yo, rmse2, excluded=None):
+        p = _lib.ptr
+        z = Zones(self.nzones, p(self.zone_ptr), p(self.zpos), p(self.corr_len),
+                  p(self.max_len), int(self.metric), int(self.loctype))
+        o = Obs(len(yo), p(self.ox), p(self.oy), p(self.oz), p(self.ot), p(yo), p(rmse2),
+                p(excluded))
+        return z, o
+
+
+def _prep_geo(geo, dev):
+    return geo if dev else geo.host_arrays()
+
+
+def _host_vec(a, dtype=np.float64):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+def _check_dev_matrix(a, rows, cols, name):
+    if tuple(a.shape) != (rows, cols):
+        raise ValueError("%s must have shape (%d, %d)" % (name, rows, cols))
+    if cols > 1 and a.stride(0) != 1:
+        raise ValueError("%s must be column-major (stride (1, ld))" % name)
+    return a.stride(1) if cols > 1 else rows
+
+
+def loc_analysis(geo, xf, Hxf, yo, Sf, HSf, rmse2, excluded=None, localise_obs=True,
+                 amplitudes=False, ctx=None):
+    """locAnalysis with OAK's selectObservations on the GPU.
+
+    Returns (xa, Sa) or (xa, Sa, amplitudes); with device inputs the outputs
+    are new device tensors."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    dev = _is_dev(Sf)
+    g = _prep_geo(geo, dev)
+    if dev:
+        import torch
+        n, N = Sf.shape
+        m = HSf.shape[0]
+        ldS = _check_dev_matrix(Sf, n, N, "Sf")
+        ldH = _check_dev_matrix(HSf, m, N, "HSf") if m > 0 else 1
+        xa = torch.empty(n, dtype=torch.float64, device=Sf.device)
+        Sa = torch.empty((N, n), dtype=torch.float64, device=Sf.device).t()
+        ampl = torch.zeros((N, g.nzones), dtype=torch.float64, device=Sf.device).t() if amplitudes else None
+        z, o = g.structs(yo, rmse2, excluded)
+        rc = L.oakg_local_analysis(ctx.handle, C.byref(z), C.byref(o), N, xf.data_ptr(),
+                                   _lib.ptr(Hxf), Sf.data_ptr(), ldS, _lib.ptr(HSf), max(ldH, 1),
+                                   xa.data_ptr(), Sa.data_ptr(), n,
+                                   None if ampl is None else ampl.data_ptr(),
+                                   1 if localise_obs else 0, _lib.OAKG_DEVICE_PTRS)
+        ctx.check(rc)
+        return (xa, Sa, ampl.t()) if amplitudes else (xa, Sa)
+    Sf = np.asfortranarray(Sf, dtype=np.float64)
+    HSf = np.asfortranarray(HSf, dtype=np.float64)
+    n, N = Sf.shape
+    m = HSf.shape[0]
+    xf = _host_vec(xf); Hxf = _host_vec(Hxf); yo = _host_vec(yo); rmse2 = _host_vec(rmse2)
+    ex = _host_vec(excluded, np.uint8)
+    z, o = g.structs(yo, rmse2, ex)
+    xa = np.zeros(n)
+    Sa = np.zeros((n, N), order="F")
+    ampl = np.zeros((N, g.nzones), order="F") if amplitudes else None
+    rc = L.oakg_local_analysis(ctx.handle, C.byref(z), C.byref(o), N, xf.ctypes.data,
+                               Hxf.ctypes.data, Sf.ctypes.data, n, HSf.ctypes.data, max(m, 1),
+                               xa.ctypes.data, Sa.ctypes.data, n,
+                               None if ampl is None else ampl.ctypes.data,
+                               1 if localise_obs else 0, 0)
+    ctx.check(rc)
+    return (xa, Sa, ampl) if amplitudes else (xa, Sa)
+
+
+def select_observations(geo, yo, rmse2, localise_obs=True, ctx=None):
+    """selectObservations for every zone on the GPU (host arrays).
+    Returns (sel_ptr, sel_idx, sel_w) with each zone's list sorted by
+    observation index."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    g = geo.host_arrays()
+    yo = _host_vec(yo); rmse2 = _host_vec(rmse2)
+    z, o = g.structs(yo, rmse2)
+    nz = g.nzones
+    sel_ptr = np.zeros(nz + 1, dtype=np.int64)
+    total = C.c_int64(0)
+    ctx.check(L.oakg_select_observations(ctx.handle, C.byref(z), C.byref(o),
+                                         1 if localise_obs else 0, sel_ptr.ctypes.data, None,
+                                         None, 0, C.byref(total), 0))
+    cap = max(total.value, 1)
+    idx = np.zeros(cap, dtype=np.int32)
+    w = np.zeros(cap)
+    ctx.check(L.oakg_select_observations(ctx.handle, C.byref(z), C.byref(o),
+                                         1 if localise_obs else 0, sel_ptr.ctypes.data,
+                                         idx.ctypes.data, w.ctypes.data, cap, C.byref(total), 0))
+    idx = idx[: total.value]; w = w[: total.value]
+    for zz in range(nz):
+        a, b = sel_ptr[zz], sel_ptr[zz + 1]
+        order = np.argsort(idx[a:b], kind="stable")
+        idx[a:b] = idx[a:b][order]
+        w[a:b] = w[a:b][order]
+    return sel_ptr, idx, w
+
+
+def coo_to_csr(m, rows, cols, vals):
+    """Stable COO -> CSR by observation (SparseMatrix, matoper.F90:30-39)."""
+    L = _lib.load()
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64)
+    nnz = len(vals)
+    rp = np.zeros(m + 1, dtype=np.int64)
+    co = np.zeros(max(nnz, 1), dtype=np.int32)
+    va = np.zeros(max(nnz, 1))
+    rc = L.oakg_coo_to_csr(m, nnz, rows.ctypes.data, cols.ctypes.data, vals.ctypes.data,
+                           rp.ctypes.data, co.ctypes.data, va.ctypes.data)
+    if rc != 0:
+        raise _lib.OakError(rc, "coo_to_csr: row index out of range")
+    return rp, co[:nnz], va[:nnz]
+
+
+def partition(part):
+    """initPartition (assimilation.F90:578-641) after the relabelling of
+    :408-412.  Returns (zone_size, zone_index, inv_zone_index), 0-based."""
+    L = _lib.load()
+    part = np.ascontiguousarray(part, np.int64)
+    n = len(part)
+    zs = np.zeros(max(n, 1), np.int64); zi = np.zeros(max(n, 1), np.int64)
+    izi = np.zeros(max(n, 1), np.int64)
+    nz = L.oakg_partition(n, part.ctypes.data, zs.ctypes.data, zi.ctypes.data, izi.ctypes.data)
+    if nz < 0:
+        raise _lib.OakError(1, "partition failed")
+    return zs[:nz], zi[:n], izi[:n]
+
+
+def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.0,
+                      max_correction=np.inf, ctx=None, out=None):
+    """oak_assim for the local scheme: Ef (n x N, zone-permuted) -> Ea.
+
+    H = (row_ptr, col, val) in CSR by observation (see coo_to_csr).
+    With torch device tensors, ``out`` may give the preallocated Ea."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    dev = _is_dev(Ef)
+    g = _prep_geo(geo, dev)
+    rp, co, va = H
+    m = len(yo)
+    n, N = Ef.shape
+    maxc = float(max_correction) if np.isfinite(max_correction) else float("inf")
+    if dev:
+        import torch
+        ldE = _check_dev_matrix(Ef, n, N, "Ef")
+        Ea = out if out is not None else torch.empty((N, n), dtype=Ef.dtype, device=Ef.device).t()
+        ldA = _check_dev_matrix(Ea, n, N, "Ea")
+        z, o = g.structs(yo, rmse2, excluded)
+        h = Csr(m, n, len(va), rp.data_ptr(), co.data_ptr(), va.data_ptr())
+        fn = L.oakg_ensemble_analysis if Ef.dtype == torch.float64 else L.oakg_ensemble_analysis_f32
+        rc = fn(ctx.handle, C.byref(z), C.byref(o), C.byref(h), Hshift.data_ptr(), N,
+                Ef.data_ptr(), ldE, Ea.data_ptr(), ldA, float(inflation), maxc,
+                _lib.OAKG_DEVICE_PTRS)
+        ctx.check(rc)
+        return Ea
+    Ef = np.asfortranarray(Ef, dtype=np.float64)
+    yo = _host_vec(yo); rmse2 = _host_vec(rmse2); Hshift = _host_vec(Hshift)
+    ex = _host_vec(excluded, np.uint8)
+    rp = np.ascontiguousarray(rp, np.int64); co = np.ascontiguousarray(co, np.int32)
+    va = np.ascontiguousarray(va, np.float64)
+    z, o = g.structs(yo, rmse2, ex)
+    h = Csr(m, n, len(va), rp.ctypes.data, co.ctypes.data, va.ctypes.data)
+    Ea = np.zeros((n, N), order="F")
+    rc = L.oakg_ensemble_analysis(ctx.handle, C.byref(z), C.byref(o), C.byref(h),
+                                  Hshift.ctypes.data, N, Ef.ctypes.data, n, Ea.ctypes.data, n,
+                                  float(inflation), maxc, 0)
+    ctx.check(rc)
+    return Ea

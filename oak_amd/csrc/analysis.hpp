@@ -1,0 +1,97 @@
+// Launch interfaces of the per-zone analysis kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace oakg {
+
+enum Phase { PH_PREP = 0, PH_SELECT, PH_GRAM, PH_EIG, PH_TMAT, PH_UPDATE, PH_END, PH_COUNT };
+
+// Records an event at each phase boundary; per-phase device time is the sum
+// of the intervals that start with a mark of that phase.
+struct PhaseTimer {
+  hipStream_t st = nullptr;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> phase;
+  size_t used = 0;
+  void reset(hipStream_t s) { st = s; used = 0; phase.clear(); }
+  void mark(int ph) {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pool.push_back(e);
+    }
+    (void)hipEventRecord(pool[used], st);
+    phase.push_back(ph);
+    ++used;
+  }
+  // accumulates milliseconds per phase into ms[PH_COUNT]; call after a sync
+  void collect(double* ms) {
+    for (int i = 0; i < PH_COUNT; ++i) ms[i] = 0;
+    for (size_t i = 0; i + 1 < used; ++i) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, pool[i], pool[i + 1]) == hipSuccess) ms[phase[i]] += t;
+    }
+  }
+  ~PhaseTimer() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+struct PrepLocArgs {
+  int64_t m;
+  int N;
+  const double* HSf;
+  int64_t ldHSf;
+  const double *Hxf, *yo, *rmse2;
+  const uint8_t* excl;
+  double *HS, *innov, *rho;
+  unsigned* err;
+};
+
+struct PrepEnsArgs {
+  int64_t n, m;
+  int N;
+  const double* E;
+  int64_t ldE;
+  const int64_t* row_ptr;
+  const int32_t* col;
+  const double* val;
+  const double* Hshift;
+  const double *yo, *rmse2;
+  const uint8_t* excl;
+  double scaling;
+  double *xbar, *HEc, *HS, *innov, *rho;
+  unsigned* err;
+};
+
+struct ChunkArgs {
+  int64_t z0;
+  int nzc, N, mode;  // mode 0 = locAnalysis, 1 = ensemble
+  int64_t m;
+  const int64_t* zone_ptr;
+  const int64_t* sel_ptr;
+  const int32_t* sel_idx;
+  const double* sel_w;
+  const double *rho, *innov, *HS;
+  double *mat, *ymat, *vec, *ampl;
+  int32_t* sweeps;
+  unsigned* err;
+  double tol;
+  int max_sweeps;
+  const double* X0;  // xf (mode 0) or xbar (mode 1)
+  const double* S;   // Sf (mode 0) or E (mode 1)
+  int64_t ldS;
+  double* xa;   // mode 0
+  double* Out;  // Sa (mode 0) or Ea (mode 1)
+  int64_t ldO;
+  double scaling, infl, maxcorr;
+};
+
+void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
+void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
+void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+
+}  // namespace oakg

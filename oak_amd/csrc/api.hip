@@ -1,0 +1,750 @@
+// C ABI (include/oakg.h): device context, staging of host arrays, the zone
+// loop of locAnalysisIncrement (rrsqrt.F90:258-426) in chunks of zones, and
+// the error contract (outputs untouched on error with host pointers).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/oakg.h"
+#include "analysis.hpp"
+#include "common.hpp"
+#include "internal.hpp"
+
+struct oakg_ctx {
+  int device = 0;
+  int precision = OAKG_FP64;
+  hipStream_t stream = nullptr;
+  std::string err;
+  oakg::Workspace ws;
+  unsigned* d_err = nullptr;
+  oakg_stats stats{};
+  std::vector<int64_t> counts;
+  bool timing = false;
+  oakg::PhaseTimer timer;
+  double jacobi_tol = 0;  // 0 = default for the precision
+  int max_sweeps = 40;
+  // last call, for the lazily copied per-zone counts
+  int64_t last_nz = 0;
+  const int64_t* last_sel_ptr = nullptr;
+  int64_t* d_stats = nullptr;  // device: error word + reduced statistics
+  bool counts_ready = true;
+};
+
+namespace oakg {
+
+__global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(unsigned* p, int64_t n, unsigned v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+// Per-call statistics reduced on the device: [0] error bits, [1] zones with
+// p > 0, [2] sum p, [3] max p, [4] max sweeps.  One block.
+__global__ __launch_bounds__(1024) void k_call_stats(int64_t nz, const int64_t* __restrict__ sel_ptr,
+                                                     const int32_t* __restrict__ sweeps,
+                                                     const unsigned* __restrict__ err,
+                                                     int64_t* __restrict__ out) {
+  __shared__ int64_t sh[3][1024];
+  int64_t na = 0, pm = 0, sm = 0;
+  for (int64_t z = threadIdx.x; z < nz; z += blockDim.x) {
+    const int64_t p = sel_ptr[z + 1] - sel_ptr[z];
+    na += p > 0;
+    pm = p > pm ? p : pm;
+    if (sweeps) sm = sweeps[z] > sm ? sweeps[z] : sm;
+  }
+  const int t = threadIdx.x;
+  sh[0][t] = na; sh[1][t] = pm; sh[2][t] = sm;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      sh[0][t] += sh[0][t + s];
+      sh[1][t] = sh[1][t] > sh[1][t + s] ? sh[1][t] : sh[1][t + s];
+      sh[2][t] = sh[2][t] > sh[2][t + s] ? sh[2][t] : sh[2][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = (int64_t)*err;
+    out[1] = sh[0][0];
+    out[2] = nz > 0 ? sel_ptr[nz] - sel_ptr[0] : 0;
+    out[3] = sh[1][0];
+    out[4] = sh[2][0];
+  }
+}
+
+void fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+void fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+static void fill_f64(double* p, int64_t n, double v, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+}  // namespace oakg
+
+using namespace oakg;
+
+namespace {
+
+struct Fail {
+  int code;
+};
+
+void check(oakg_ctx* c, hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    throw Fail{e == hipErrorOutOfMemory ? OAKG_ENOMEM : OAKG_EHIP};
+  }
+}
+
+void arg_fail(oakg_ctx* c, const std::string& msg) {
+  c->err = msg;
+  throw Fail{OAKG_EARG};
+}
+
+int padded_n(int N) {
+  if (N <= 16) return 16;
+  if (N <= 32) return 32;
+  if (N <= 64) return 64;
+  return 0;
+}
+
+// Stages a host array into a workspace slot (or returns the device pointer).
+template <typename T>
+const T* stage(oakg_ctx* c, Workspace::Slot s, const T* p, int64_t count, bool dev) {
+  if (!p || count <= 0) return p;
+  if (dev) return p;
+  T* d = (T*)c->ws.get(s, sizeof(T) * count);
+  if (!d) check(c, hipErrorOutOfMemory, "staging");
+  check(c, hipMemcpyAsync(d, p, sizeof(T) * count, hipMemcpyHostToDevice, c->stream), "H2D");
+  return d;
+}
+
+// Stages a column-major rows x cols block with leading dimension ld into a
+// dense (ld = rows) device copy.
+const double* stage2d(oakg_ctx* c, Workspace::Slot s, const double* p, int64_t rows, int cols,
+                      int64_t ld, bool dev, int64_t* ld_out) {
+  if (dev) {
+    *ld_out = ld;
+    return p;
+  }
+  double* d = (double*)c->ws.get(s, sizeof(double) * std::max<int64_t>(rows, 1) * cols);
+  if (!d) check(c, hipErrorOutOfMemory, "staging");
+  if (rows > 0)
+    check(c,
+          hipMemcpy2DAsync(d, sizeof(double) * rows, p, sizeof(double) * ld, sizeof(double) * rows,
+                           cols, hipMemcpyHostToDevice, c->stream),
+          "H2D 2D");
+  *ld_out = rows;
+  return d;
+}
+
+struct Geo {
+  ZonesDev zd;
+  ObsDev od;
+};
+
+Geo stage_geo(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, bool dev, bool need_geo) {
+  Geo g{};
+  const int64_t nz = zones ? zones->nzones : 0;
+  const int64_t m = obs->m;
+  if (zones) {
+    g.zd.nzones = nz;
+    g.zd.metric = zones->metric;
+    g.zd.loctype = zones->loctype;
+    g.zd.zone_ptr = stage(c, Workspace::Z_PTR, zones->zone_ptr, nz + 1, dev);
+    if (need_geo) {
+      g.zd.zpos = stage(c, Workspace::Z_POS, zones->zpos, nz * 4, dev);
+      g.zd.corr = stage(c, Workspace::Z_CORR, zones->corr_len, nz, dev);
+      g.zd.maxlen = stage(c, Workspace::Z_MAXLEN, zones->max_len, nz, dev);
+    }
+  }
+  g.od.m = m;
+  if (need_geo) {
+    g.od.x = stage(c, Workspace::O_X, obs->x, m, dev);
+    g.od.y = stage(c, Workspace::O_Y, obs->y, m, dev);
+    g.od.z = stage(c, Workspace::O_Z, obs->z, m, dev);
+    g.od.t = stage(c, Workspace::O_T, obs->t, m, dev);
+  }
+  g.od.yo = stage(c, Workspace::O_YO, obs->yo, m, dev);
+  g.od.rmse2 = stage(c, Workspace::O_RMSE2, obs->rmse2, m, dev);
+  g.od.excluded = stage(c, Workspace::O_EXCL, obs->excluded, m, dev);
+  return g;
+}
+
+void validate_zones_host(oakg_ctx* c, const oakg_zones* z, int64_t n_expect, bool need_geo) {
+  if (!z || z->nzones < 0 || !z->zone_ptr) arg_fail(c, "zones: NULL or negative count");
+  if (need_geo) {
+    if (z->nzones > 0 && (!z->zpos || !z->corr_len || !z->max_len))
+      arg_fail(c, "zones: zpos/corr_len/max_len required");
+    if (z->metric < 0 || z->metric > 2) arg_fail(c, "zones: metric must be 0, 1 or 2");
+    if (z->loctype < 1 || z->loctype > 3) arg_fail(c, "zones: loctype must be 1, 2 or 3");
+  }
+  (void)n_expect;
+}
+
+void validate_zone_ptr_host(oakg_ctx* c, int64_t nz, const int64_t* zp) {
+  if (zp[0] != 0) arg_fail(c, "zone_ptr[0] must be 0");
+  for (int64_t z = 0; z < nz; ++z)
+    if (zp[z + 1] < zp[z]) arg_fail(c, "zone_ptr must be non-decreasing");
+}
+
+void validate_obs(oakg_ctx* c, const oakg_obs* o, bool need_geo, int loctype) {
+  if (!o || o->m < 0) arg_fail(c, "obs: NULL or negative count");
+  if (o->m > 0 && (!o->yo || !o->rmse2)) arg_fail(c, "obs: yo and rmse2 required");
+  if (need_geo && o->m > 0) {
+    if (loctype == 1 && (!o->x || !o->y)) arg_fail(c, "obs: x and y required for loctype 1");
+    if (loctype == 2 && !o->z) arg_fail(c, "obs: z required for loctype 2");
+    if (loctype == 3 && !o->t) arg_fail(c, "obs: t required for loctype 3");
+  }
+  if (o->m > INT32_MAX) arg_fail(c, "obs: m above 2^31-1");
+}
+
+void begin_call(oakg_ctx* c) {
+  c->err.clear();
+  check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (!c->d_err) {
+    check(c, hipMalloc(&c->d_err, sizeof(unsigned)), "hipMalloc");
+    check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
+  }
+  fill_u32(c->d_err, 1, 0u, c->stream);
+  memset(&c->stats, 0, sizeof c->stats);
+  c->counts_ready = true;
+  c->counts.clear();
+  if (c->timing) c->timer.reset(c->stream);
+}
+
+// Ends a call: reduces the statistics on the device, brings them back with
+// the error word in one copy, synchronizes, and turns device errors into
+// return codes.  sel_ptr may be NULL (no zone loop in the call).
+void finish_call(oakg_ctx* c, int64_t nz, const int64_t* sel_ptr, const int32_t* sweeps) {
+  int64_t h[5] = {0, 0, 0, 0, 0};
+  hipLaunchKernelGGL(k_call_stats, dim3(1), dim3(1024), 0, c->stream, sel_ptr ? nz : 0,
+                     sel_ptr, sweeps, c->d_err, c->d_stats);
+  check(c, hipGetLastError(), "stats launch");
+  check(c, hipMemcpyAsync(h, c->d_stats, sizeof h, hipMemcpyDeviceToHost, c->stream), "D2H stats");
+  check(c, hipStreamSynchronize(c->stream), "sync");
+  c->stats.zones_analysed = h[1];
+  c->stats.obs_selected = h[2];
+  c->stats.pmax = h[3];
+  c->stats.sweeps_max = (int32_t)h[4];
+  const unsigned herr = (unsigned)h[0];
+  if (herr & OAKG_DERR_RMSE) arg_fail(c, "observation error variance rmse^2 <= 0 or not finite");
+  if (herr & OAKG_DERR_IDX) arg_fail(c, "index out of range in H or in the selection");
+  if (herr & OAKG_DERR_NAN) {
+    c->err = "NaN in the analysis amplitudes (rrsqrt.F90:145-149)";
+    throw Fail{OAKG_ENAN};
+  }
+}
+
+// Records the phase times of the call and where its per-zone counts are;
+// the counts are copied to the host only when asked for (oakg_get_counts).
+void collect_stats(oakg_ctx* c, int64_t nz, const int64_t* d_sel_ptr, double ms_total) {
+  c->last_nz = nz;
+  c->last_sel_ptr = d_sel_ptr;
+  c->counts_ready = false;
+  c->stats.zones = nz;
+  c->stats.ms_total = ms_total;
+  if (c->timing) {
+    double ms[PH_COUNT];
+    c->timer.collect(ms);
+    c->stats.ms_prep = ms[PH_PREP];
+    c->stats.ms_select = ms[PH_SELECT];
+    c->stats.ms_gram = ms[PH_GRAM];
+    c->stats.ms_eig = ms[PH_EIG] + ms[PH_TMAT];
+    c->stats.ms_update = ms[PH_UPDATE];
+  }
+}
+
+void materialize_counts(oakg_ctx* c) {
+  if (c->counts_ready) return;
+  const int64_t nz = c->last_nz;
+  std::vector<int64_t> ptr(nz + 1);
+  check(c, hipSetDevice(c->device), "hipSetDevice");
+  check(c, hipMemcpy(ptr.data(), c->last_sel_ptr, sizeof(int64_t) * (nz + 1),
+                     hipMemcpyDeviceToHost), "D2H counts");
+  c->counts.resize(nz);
+  for (int64_t z = 0; z < nz; ++z) c->counts[z] = ptr[z + 1] - ptr[z];
+  c->counts_ready = true;
+}
+
+// Runs the chunked zone loop (gram, eig, tmat, update) for all zones.
+void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_all) {
+  const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 4 * NP);
+  int64_t zc = std::min<int64_t>(nz, std::max<int64_t>(1, (int64_t)((size_t)3 << 30) / per_zone));
+  zc = std::min<int64_t>(zc, 1 << 20);
+  double* mat = (double*)c->ws.get(Workspace::MAT, sizeof(double) * (size_t)NP * NP * zc);
+  double* ymat = (double*)c->ws.get(Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zc);
+  double* vec = (double*)c->ws.get(Workspace::VEC, sizeof(double) * (size_t)4 * NP * zc);
+  if (!mat || !ymat || !vec) check(c, hipErrorOutOfMemory, "workspace");
+  for (int64_t z0 = 0; z0 < nz; z0 += zc) {
+    ChunkArgs A = base;
+    A.z0 = z0;
+    A.nzc = (int)std::min<int64_t>(zc, nz - z0);
+    A.mat = mat;
+    A.ymat = ymat;
+    A.vec = vec;
+    A.sweeps = sweeps_all + z0;
+    launch_chunk(NP, A, c->stream, c->timing ? &c->timer : nullptr);
+    check(c, hipGetLastError(), "chunk launch");
+  }
+}
+
+template <typename F>
+int guarded(oakg_ctx* c, F f) {
+  if (!c) return OAKG_EARG;
+  try {
+    return f();
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (...) {
+    c->err = "unexpected host exception";
+    return OAKG_EHIP;
+  }
+}
+
+// Common body of the two locAnalysis entry points.
+int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr,
+                        const int64_t* zone_ptr_csr, const int64_t* sel_ptr_in,
+                        const int32_t* sel_idx_in, const double* sel_w_in, const oakg_obs* obs,
+                        int32_t N, const double* xf, const double* Hxf, const double* Sf,
+                        int64_t ldSf, const double* HSf, int64_t ldHSf, double* xa, double* Sa,
+                        int64_t ldSa, double* ampl, int32_t localise, uint32_t flags) {
+  const bool dev = flags & OAKG_DEVICE_PTRS;
+  const bool csr = sel_ptr_in != nullptr;
+  if (N < 1 || N > OAKG_MAX_N) arg_fail(c, "N out of range");
+  const int NP = padded_n(N);
+  if (!NP) {
+    c->err = "N above 64 is not supported by this build yet";
+    return OAKG_EUNSUP;
+  }
+  if (!csr) validate_zones_host(c, zones, 0, true);
+  validate_obs(c, obs, !csr, zones ? zones->loctype : 1);
+  const int64_t nz = csr ? nzones_csr : zones->nzones;
+  const int64_t* zp_host = csr ? zone_ptr_csr : zones->zone_ptr;
+  if (nz < 0 || !zp_host) arg_fail(c, "zone_ptr required");
+  if (!xf || !Sf || !xa || !Sa || (obs->m > 0 && (!Hxf || !HSf)))
+    arg_fail(c, "xf, Sf, xa, Sa, Hxf and HSf are required");
+  int64_t n = 0;
+  if (!dev) {
+    validate_zone_ptr_host(c, nz, zp_host);
+    n = zp_host[nz];
+  } else {
+    int64_t last = 0;
+    if (nz > 0)
+      check(c, hipMemcpy(&last, zp_host + nz, sizeof(int64_t), hipMemcpyDeviceToHost), "D2H n");
+    n = last;
+  }
+  const int64_t m = obs->m;
+  if (ldSf < n || ldSa < n || (m > 0 && ldHSf < m)) arg_fail(c, "leading dimension too small");
+  begin_call(c);
+  if (c->timing) c->timer.mark(PH_PREP);
+  hipEvent_t e0, e1;
+  check(c, hipEventCreate(&e0), "event");
+  check(c, hipEventCreate(&e1), "event");
+  check(c, hipEventRecord(e0, c->stream), "event");
+
+  oakg_zones zz{};
+  if (csr) {
+    zz.nzones = nz;
+    zz.zone_ptr = zone_ptr_csr;
+  } else {
+    zz = *zones;
+  }
+  Geo g = stage_geo(c, &zz, obs, dev, !csr);
+  int64_t ld_sf = 0, ld_hs = 0;
+  const double* dSf = stage2d(c, Workspace::IN_S, Sf, n, N, ldSf, dev, &ld_sf);
+  const double* dHSf = stage2d(c, Workspace::IN_HSF, HSf, m, N, ldHSf, dev, &ld_hs);
+  const double* dxf = stage(c, Workspace::IN_XF, xf, n, dev);
+  const double* dHxf = stage(c, Workspace::IN_HXF, Hxf, m, dev);
+
+  double* HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+  double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+  double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+  if (!HS || !innov || !rho) check(c, hipErrorOutOfMemory, "workspace");
+  PrepLocArgs P{m, N, dHSf, ld_hs, dHxf, g.od.yo, g.od.rmse2, g.od.excluded, HS, innov, rho,
+                c->d_err};
+  launch_prep_loc(NP, P, c->stream);
+  check(c, hipGetLastError(), "prep");
+
+  if (c->timing) c->timer.mark(PH_SELECT);
+  SelDev sel{};
+  int64_t total = 0;
+  if (csr) {
+    // caller-supplied selection
+    const int64_t* dptr = stage(c, Workspace::U_SEL_PTR, sel_ptr_in, nz + 1, dev);
+    int64_t tot = 0;
+    if (!dev) {
+      if (sel_ptr_in[0] != 0) arg_fail(c, "sel_ptr[0] must be 0");
+      for (int64_t z = 0; z < nz; ++z)
+        if (sel_ptr_in[z + 1] < sel_ptr_in[z]) arg_fail(c, "sel_ptr must be non-decreasing");
+      tot = sel_ptr_in[nz];
+      for (int64_t e = 0; e < tot; ++e)
+        if (sel_idx_in[e] < 0 || sel_idx_in[e] >= m) arg_fail(c, "sel_idx out of range");
+      if (!localise)
+        for (int64_t z = 0; z < nz; ++z) {
+          int64_t pz = sel_ptr_in[z + 1] - sel_ptr_in[z];
+          if (pz != 0 && pz != m) arg_fail(c, "localise_obs = 0 needs all or no observations per zone");
+        }
+    } else if (nz > 0) {
+      check(c, hipMemcpy(&tot, sel_ptr_in + nz, sizeof(int64_t), hipMemcpyDeviceToHost), "D2H");
+    }
+    sel.ptr = dptr;
+    sel.idx = stage(c, Workspace::U_SEL_IDX, sel_idx_in, tot, dev);
+    sel.w = stage(c, Workspace::U_SEL_W, sel_w_in, tot, dev);
+    total = tot;
+  } else {
+    check(c, run_selection(c->ws, g.zd, g.od, localise, c->stream, c->d_err, sel, &total),
+          "selection");
+  }
+
+  double* d_ampl = nullptr;
+  if (ampl) {
+    d_ampl = dev ? ampl : (double*)c->ws.get(Workspace::OUT_AMPL, sizeof(double) * (size_t)N * std::max<int64_t>(nz, 1));
+    if (!d_ampl) check(c, hipErrorOutOfMemory, "workspace");
+    fill_f64(d_ampl, (int64_t)N * nz, 0.0, c->stream);
+  }
+  double* dxa = dev ? xa : (double*)c->ws.get(Workspace::OUT_XA, sizeof(double) * std::max<int64_t>(n, 1));
+  double* dSa = dev ? Sa : (double*)c->ws.get(Workspace::OUT_S, sizeof(double) * std::max<int64_t>(n, 1) * N);
+  const int64_t ld_sa = dev ? ldSa : n;
+  if (!dxa || !dSa) check(c, hipErrorOutOfMemory, "workspace");
+  int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+  if (!sweeps) check(c, hipErrorOutOfMemory, "workspace");
+
+  ChunkArgs A{};
+  A.N = N;
+  A.mode = 0;
+  A.m = m;
+  A.zone_ptr = g.zd.zone_ptr;
+  A.sel_ptr = sel.ptr;
+  A.sel_idx = sel.idx;
+  A.sel_w = sel.w;
+  A.rho = rho;
+  A.innov = innov;
+  A.HS = HS;
+  A.ampl = (ampl && !localise) ? d_ampl : nullptr;
+  A.err = c->d_err;
+  A.tol = c->jacobi_tol > 0 ? c->jacobi_tol : 4.0 * NP * 2.220446049250313e-16;
+  A.max_sweeps = c->max_sweeps;
+  A.X0 = dxf;
+  A.S = dSf;
+  A.ldS = ld_sf;
+  A.xa = dxa;
+  A.Out = dSa;
+  A.ldO = ld_sa;
+  A.scaling = 1.0;
+  A.infl = 1.0;
+  A.maxcorr = HUGE_VAL;
+  run_zones(c, NP, A, nz, sweeps);
+  if (c->timing) c->timer.mark(PH_END);
+  check(c, hipEventRecord(e1, c->stream), "event");
+  finish_call(c, nz, sel.ptr, sweeps);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  collect_stats(c, nz, sel.ptr, ms);
+  (void)total;
+  if (!dev) {
+    check(c, hipMemcpy(xa, dxa, sizeof(double) * n, hipMemcpyDeviceToHost), "D2H xa");
+    if (n > 0)
+      check(c,
+            hipMemcpy2D(Sa, sizeof(double) * ldSa, dSa, sizeof(double) * n, sizeof(double) * n, N,
+                        hipMemcpyDeviceToHost),
+            "D2H Sa");
+    if (ampl)
+      check(c, hipMemcpy(ampl, d_ampl, sizeof(double) * (size_t)N * nz, hipMemcpyDeviceToHost),
+            "D2H ampl");
+  }
+  return OAKG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oakg_version(void) { return 10000; }
+
+int oakg_create(oakg_ctx** out, int device, int precision) {
+  if (!out) return OAKG_EARG;
+  *out = nullptr;
+  if (precision != OAKG_FP64 && precision != OAKG_MIXED) return OAKG_EARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return OAKG_EHIP;
+  if (hipSetDevice(device) != hipSuccess) return OAKG_EHIP;
+  oakg_ctx* c = new (std::nothrow) oakg_ctx;
+  if (!c) return OAKG_ENOMEM;
+  c->device = device;
+  c->precision = precision;
+  *out = c;
+  return OAKG_OK;
+}
+
+int oakg_destroy(oakg_ctx* c) {
+  if (!c) return OAKG_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->d_err) (void)hipFree(c->d_err);
+  if (c->d_stats) (void)hipFree(c->d_stats);
+  c->ws.release();
+  delete c;
+  return OAKG_OK;
+}
+
+const char* oakg_last_error(const oakg_ctx* c) { return c ? c->err.c_str() : "NULL context"; }
+
+int oakg_set_stream(oakg_ctx* c, void* stream) {
+  if (!c) return OAKG_EARG;
+  c->stream = (hipStream_t)stream;
+  return OAKG_OK;
+}
+
+int oakg_set_timing(oakg_ctx* c, int on) {
+  if (!c) return OAKG_EARG;
+  c->timing = on != 0;
+  return OAKG_OK;
+}
+
+int oakg_get_stats(oakg_ctx* c, oakg_stats* out) {
+  if (!c || !out) return OAKG_EARG;
+  *out = c->stats;
+  return OAKG_OK;
+}
+
+int oakg_get_counts(oakg_ctx* c, int64_t* counts, int64_t nzones) {
+  if (!c || !counts) return OAKG_EARG;
+  return guarded(c, [&] {
+    materialize_counts(c);
+    if (nzones != (int64_t)c->counts.size()) arg_fail(c, "nzones differs from the last call");
+    memcpy(counts, c->counts.data(), sizeof(int64_t) * nzones);
+    return OAKG_OK;
+  });
+}
+
+int oakg_local_analysis(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, int32_t N,
+                        const double* xf, const double* Hxf, const double* Sf, int64_t ldSf,
+                        const double* HSf, int64_t ldHSf, double* xa, double* Sa, int64_t ldSa,
+                        double* amplitudes, int32_t localise_obs, uint32_t flags) {
+  return guarded(c, [&] {
+    if (!zones || !obs) arg_fail(c, "zones and obs are required");
+    return local_analysis_impl(c, zones, 0, nullptr, nullptr, nullptr, nullptr, obs, N, xf, Hxf,
+                               Sf, ldSf, HSf, ldHSf, xa, Sa, ldSa, amplitudes, localise_obs, flags);
+  });
+}
+
+int oakg_local_analysis_csr(oakg_ctx* c, int64_t nzones, const int64_t* zone_ptr,
+                            const int64_t* sel_ptr, const int32_t* sel_idx, const double* sel_w,
+                            const oakg_obs* obs, int32_t N, const double* xf, const double* Hxf,
+                            const double* Sf, int64_t ldSf, const double* HSf, int64_t ldHSf,
+                            double* xa, double* Sa, int64_t ldSa, double* amplitudes,
+                            int32_t localise_obs, uint32_t flags) {
+  return guarded(c, [&] {
+    if (!obs || !sel_ptr || !zone_ptr) arg_fail(c, "obs, zone_ptr and sel_ptr are required");
+    return local_analysis_impl(c, nullptr, nzones, zone_ptr, sel_ptr, sel_idx, sel_w, obs, N, xf,
+                               Hxf, Sf, ldSf, HSf, ldHSf, xa, Sa, ldSa, amplitudes, localise_obs,
+                               flags);
+  });
+}
+
+int oakg_select_observations(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs,
+                             int32_t localise_obs, int64_t* sel_ptr, int32_t* sel_idx,
+                             double* sel_w, int64_t cap, int64_t* total, uint32_t flags) {
+  return guarded(c, [&] {
+    const bool dev = flags & OAKG_DEVICE_PTRS;
+    validate_zones_host(c, zones, 0, true);
+    validate_obs(c, obs, true, zones->loctype);
+    if (!sel_ptr || !total) arg_fail(c, "sel_ptr and total are required");
+    begin_call(c);
+    Geo g = stage_geo(c, zones, obs, dev, true);
+    SelDev sel{};
+    int64_t tot = 0;
+    check(c, run_selection(c->ws, g.zd, g.od, localise_obs, c->stream, c->d_err, sel, &tot),
+          "selection");
+    finish_call(c, zones->nzones, sel.ptr, nullptr);
+    const int64_t nz = zones->nzones;
+    hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    check(c, hipMemcpy(sel_ptr, sel.ptr, sizeof(int64_t) * (nz + 1), k), "copy sel_ptr");
+    const int64_t ncopy = std::min(tot, cap);
+    if (sel_idx && ncopy > 0) check(c, hipMemcpy(sel_idx, sel.idx, sizeof(int32_t) * ncopy, k), "copy idx");
+    if (sel_w && ncopy > 0) check(c, hipMemcpy(sel_w, sel.w, sizeof(double) * ncopy, k), "copy w");
+    *total = tot;
+    return OAKG_OK;
+  });
+}
+
+int oakg_ensemble_analysis(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs,
+                           const oakg_csr* H, const double* Hshift, int32_t N, const double* Ef,
+                           int64_t ldEf, double* Ea, int64_t ldEa, double inflation,
+                           double max_correction, uint32_t flags) {
+  return guarded(c, [&] {
+    const bool dev = flags & OAKG_DEVICE_PTRS;
+    if (!zones || !obs || !H || !Ef || !Ea) arg_fail(c, "zones, obs, H, Ef and Ea are required");
+    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+    const int NP = padded_n(N);
+    if (!NP) {
+      c->err = "N above 64 is not supported by this build yet";
+      return OAKG_EUNSUP;
+    }
+    validate_zones_host(c, zones, 0, true);
+    validate_obs(c, obs, true, zones->loctype);
+    const int64_t nz = zones->nzones, m = obs->m;
+    if (H->m != m) arg_fail(c, "H->m must equal obs->m");
+    if (m > 0 && (!H->row_ptr || (H->nnz > 0 && (!H->col || !H->val)) || !Hshift))
+      arg_fail(c, "H arrays and Hshift are required");
+    int64_t n = 0;
+    if (!dev) {
+      validate_zone_ptr_host(c, nz, zones->zone_ptr);
+      n = zones->zone_ptr[nz];
+      if (H->row_ptr[0] != 0 || H->row_ptr[m] != H->nnz) arg_fail(c, "H->row_ptr inconsistent");
+      for (int64_t e = 0; e < H->nnz; ++e)
+        if (H->col[e] < 0 || H->col[e] >= n) arg_fail(c, "H column out of range");
+    } else if (nz > 0) {
+      check(c, hipMemcpy(&n, zones->zone_ptr + nz, sizeof(int64_t), hipMemcpyDeviceToHost), "D2H n");
+    }
+    if (H->n != n) arg_fail(c, "H->n must equal the state size");
+    if (ldEf < n || ldEa < n) arg_fail(c, "leading dimension too small");
+    if (!(inflation > 0) || !(max_correction > 0)) arg_fail(c, "inflation and max_correction must be > 0");
+    begin_call(c);
+    if (c->timing) c->timer.mark(PH_PREP);
+    hipEvent_t e0, e1;
+    check(c, hipEventCreate(&e0), "event");
+    check(c, hipEventCreate(&e1), "event");
+    check(c, hipEventRecord(e0, c->stream), "event");
+    Geo g = stage_geo(c, zones, obs, dev, true);
+    int64_t ld_e = 0;
+    const double* dE = stage2d(c, Workspace::IN_S, Ef, n, N, ldEf, dev, &ld_e);
+    const int64_t* drp = stage(c, Workspace::H_RP, H->row_ptr, m + 1, dev);
+    const int32_t* dcol = stage(c, Workspace::H_COL, H->col, H->nnz, dev);
+    const double* dval = stage(c, Workspace::H_VAL, H->val, H->nnz, dev);
+    const double* dHs = stage(c, Workspace::H_SHIFT, Hshift, m, dev);
+    double* xbar = (double*)c->ws.get(Workspace::XBAR, sizeof(double) * std::max<int64_t>(n, 1));
+    double* HEc = (double*)c->ws.get(Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
+    double* HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+    double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+    double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+    if (!xbar || !HEc || !HS || !innov || !rho) check(c, hipErrorOutOfMemory, "workspace");
+    const double scaling = sqrt((double)N - 1.0);
+    PrepEnsArgs P{n, m, N, dE, ld_e, drp, dcol, dval, dHs, g.od.yo, g.od.rmse2, g.od.excluded,
+                  scaling, xbar, HEc, HS, innov, rho, c->d_err};
+    launch_prep_ens(NP, P, c->stream);
+    check(c, hipGetLastError(), "prep");
+    if (c->timing) c->timer.mark(PH_SELECT);
+    SelDev sel{};
+    int64_t total = 0;
+    check(c, run_selection(c->ws, g.zd, g.od, 1, c->stream, c->d_err, sel, &total), "selection");
+    double* dEa = dev ? Ea : (double*)c->ws.get(Workspace::OUT_S, sizeof(double) * std::max<int64_t>(n, 1) * N);
+    const int64_t ld_ea = dev ? ldEa : n;
+    int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+    if (!dEa || !sweeps) check(c, hipErrorOutOfMemory, "workspace");
+    ChunkArgs A{};
+    A.N = N;
+    A.mode = 1;
+    A.m = m;
+    A.zone_ptr = g.zd.zone_ptr;
+    A.sel_ptr = sel.ptr;
+    A.sel_idx = sel.idx;
+    A.sel_w = sel.w;
+    A.rho = rho;
+    A.innov = innov;
+    A.HS = HS;
+    A.ampl = nullptr;
+    A.err = c->d_err;
+    A.tol = c->jacobi_tol > 0 ? c->jacobi_tol : 4.0 * NP * 2.220446049250313e-16;
+    A.max_sweeps = c->max_sweeps;
+    A.X0 = xbar;
+    A.S = dE;
+    A.ldS = ld_e;
+    A.xa = nullptr;
+    A.Out = dEa;
+    A.ldO = ld_ea;
+    A.scaling = scaling;
+    A.infl = inflation;
+    A.maxcorr = max_correction;
+    run_zones(c, NP, A, nz, sweeps);
+    if (c->timing) c->timer.mark(PH_END);
+    check(c, hipEventRecord(e1, c->stream), "event");
+    finish_call(c, nz, sel.ptr, sweeps);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    collect_stats(c, nz, sel.ptr, ms);
+    if (!dev && n > 0)
+      check(c,
+            hipMemcpy2D(Ea, sizeof(double) * ldEa, dEa, sizeof(double) * n, sizeof(double) * n, N,
+                        hipMemcpyDeviceToHost),
+            "D2H Ea");
+    return OAKG_OK;
+  });
+}
+
+int oakg_ensemble_analysis_f32(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs,
+                               const oakg_csr* H, const float* Hshift, int32_t N, const float* Ef,
+                               int64_t ldEf, float* Ea, int64_t ldEa, double inflation,
+                               double max_correction, uint32_t flags) {
+  return guarded(c, [&] {
+    (void)zones; (void)obs; (void)H; (void)Hshift; (void)N; (void)Ef; (void)ldEf; (void)Ea;
+    (void)ldEa; (void)inflation; (void)max_correction; (void)flags;
+    c->err = "mixed-precision path not built yet";
+    return OAKG_EUNSUP;
+  });
+}
+
+int oakg_coo_to_csr(int64_t m, int64_t nnz, const int32_t* row, const int32_t* col,
+                    const double* val, int64_t* row_ptr, int32_t* col_out, double* val_out) {
+  if (m < 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!row || !col || !val || !col_out || !val_out)))
+    return OAKG_EARG;
+  for (int64_t i = 0; i <= m; ++i) row_ptr[i] = 0;
+  for (int64_t e = 0; e < nnz; ++e) {
+    if (row[e] < 0 || row[e] >= m) return OAKG_EARG;
+    row_ptr[row[e] + 1]++;
+  }
+  for (int64_t i = 0; i < m; ++i) row_ptr[i + 1] += row_ptr[i];
+  std::vector<int64_t> pos(row_ptr, row_ptr + m);
+  for (int64_t e = 0; e < nnz; ++e) {
+    int64_t p = pos[row[e]]++;
+    col_out[p] = col[e];
+    val_out[p] = val[e];
+  }
+  return OAKG_OK;
+}
+
+int64_t oakg_partition(int64_t n, const int64_t* partition, int64_t* zone_size,
+                       int64_t* zone_index, int64_t* inv_zone_index) {
+  if (n < 0 || (n > 0 && (!partition || !zone_size || !zone_index || !inv_zone_index))) return -1;
+  std::vector<int64_t> u(partition, partition + n);
+  std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  const int64_t nz = (int64_t)u.size();
+  std::vector<int64_t> lab(n);
+  for (int64_t i = 0; i < n; ++i)
+    lab[i] = std::lower_bound(u.begin(), u.end(), partition[i]) - u.begin();
+  for (int64_t z = 0; z < nz; ++z) zone_size[z] = 0;
+  for (int64_t i = 0; i < n; ++i) zone_size[lab[i]]++;
+  std::vector<int64_t> ci(nz);
+  int64_t acc = 0;
+  for (int64_t z = 0; z < nz; ++z) { ci[z] = acc; acc += zone_size[z]; }
+  for (int64_t i = 0; i < n; ++i) {
+    zone_index[ci[lab[i]]] = i;
+    inv_zone_index[i] = ci[lab[i]];
+    ci[lab[i]]++;
+  }
+  return nz;
+}
+
+}  // extern "C"

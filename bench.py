@@ -126,6 +126,10 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--calib-only", action="store_true",
+                    help="run only the 1 GiB device copy and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -135,6 +139,14 @@ def main():
     from oak_amd import Context, OAKG_FP64, OAKG_MIXED
     from oak_amd.assimilation import ensemble_analysis
     device = torch.device("cuda:0")
+    if args.calib_only:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=device)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# calib copy: 1073741824 bytes read, 1073741824 bytes written", file=sys.stderr)
+        return
     t0 = time.time()
     DP, desc = build_problem(args.config, device)
     P = DP.host
@@ -192,6 +204,14 @@ def main():
         "roofline": roof,
     }
     print("# timed %d steps: %.3f ms/step" % (args.steps, ms), file=sys.stderr, flush=True)
+    if args.calib:
+        # a plain copy of known size: 1 GiB read + 1 GiB written
+        src = torch.ones(1 << 27, dtype=torch.float64, device=device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# calib copy: 1073741824 bytes read, 1073741824 bytes written", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(DP)
     print(json.dumps(out), flush=True)

@@ -107,7 +107,10 @@ typedef struct {
   int32_t sweeps_max;      /* most Jacobi sweeps used by one zone                */
   int32_t reserved;
   double ms_total;         /* device time of the whole call                      */
-  double ms_prep, ms_select, ms_gram, ms_eig, ms_update; /* per phase (timing on) */
+  /* device time per phase, recorded only with oakg_set_timing(ctx, 1):
+     prep (H·E, anomalies), select, gram (S^T R^-1 S), eig (eigensolve),
+     tmat (transform T), update (Xa) */
+  double ms_prep, ms_select, ms_gram, ms_eig, ms_tmat, ms_update;
 } oakg_stats;
 
 int oakg_version(void);

@@ -266,7 +266,8 @@ void collect_stats(oakg_ctx* c, int64_t nz, const int64_t* d_sel_ptr, double ms_
     c->stats.ms_prep = ms[PH_PREP];
     c->stats.ms_select = ms[PH_SELECT];
     c->stats.ms_gram = ms[PH_GRAM];
-    c->stats.ms_eig = ms[PH_EIG] + ms[PH_TMAT];
+    c->stats.ms_eig = ms[PH_EIG];
+    c->stats.ms_tmat = ms[PH_TMAT];
     c->stats.ms_update = ms[PH_UPDATE];
   }
 }

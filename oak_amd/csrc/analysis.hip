@@ -23,7 +23,7 @@ namespace oakg {
 // ---------------------------------------------------------------------------
 // locAnalysis mode: HSf (m x N, col-major) -> HS row-major m x NP (zero padded);
 // innov = yo - Hxf; rho = 1/rmse^2 (0 for excluded observations).
-template <int NP>
+template <int NP, int TR = (NP > 64 ? 32 : 64)>
 __global__ __launch_bounds__(256) void k_prep_loc(int64_t m, int N, const double* __restrict__ HSf,
                                                   int64_t ld, const double* __restrict__ Hxf,
                                                   const double* __restrict__ yo,
@@ -33,14 +33,14 @@ __global__ __launch_bounds__(256) void k_prep_loc(int64_t m, int N, const double
                                                   double* __restrict__ innov,
                                                   double* __restrict__ rho,
                                                   unsigned* __restrict__ err) {
-  __shared__ double tile[64][NP + 1];
-  const int64_t l0 = (int64_t)blockIdx.x * 64;
-  for (int e = threadIdx.x; e < 64 * NP; e += 256) {
-    int k = e / 64, li = e % 64;
+  __shared__ double tile[TR][NP + 1];
+  const int64_t l0 = (int64_t)blockIdx.x * TR;
+  for (int e = threadIdx.x; e < TR * NP; e += 256) {
+    int k = e / TR, li = e % TR;
     int64_t l = l0 + li;
     tile[li][k] = (l < m && k < N) ? HSf[l + (int64_t)k * ld] : 0.0;
   }
-  if (threadIdx.x < 64) {
+  if (threadIdx.x < TR) {
     int64_t l = l0 + threadIdx.x;
     if (l < m) {
       double r2 = rmse2[l];
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_prep_loc(int64_t m, int N, const double
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * NP; e += 256) {
+  for (int e = threadIdx.x; e < TR * NP; e += 256) {
     int li = e / NP, k = e % NP;
     int64_t l = l0 + li;
     if (l < m) HS[l * NP + k] = tile[li][k];
@@ -92,7 +92,7 @@ __global__ void k_spmm_obs(int64_t m, int64_t n, int N, const int64_t* __restric
 
 // Ensemble mode, observation anomalies: ybar = sum(HE,2)/N, HSf = (HE - ybar)/sqrt(N-1)
 // (assimilation.F90:3128-3134), written row-major m x NP; innov = yo - ybar.
-template <int NP>
+template <int NP, int TR = (NP > 64 ? 32 : 64)>
 __global__ __launch_bounds__(256) void k_obs_anom(int64_t m, int N, const double* __restrict__ HEc,
                                                   const double* __restrict__ yo,
                                                   const double* __restrict__ rmse2,
@@ -102,16 +102,16 @@ __global__ __launch_bounds__(256) void k_obs_anom(int64_t m, int N, const double
                                                   double* __restrict__ rho,
                                                   unsigned* __restrict__ err) {
 #pragma clang fp contract(off)
-  __shared__ double tile[64][NP + 1];
-  __shared__ double ybar[64];
-  const int64_t l0 = (int64_t)blockIdx.x * 64;
-  for (int e = threadIdx.x; e < 64 * NP; e += 256) {
-    int k = e / 64, li = e % 64;
+  __shared__ double tile[TR][NP + 1];
+  __shared__ double ybar[TR];
+  const int64_t l0 = (int64_t)blockIdx.x * TR;
+  for (int e = threadIdx.x; e < TR * NP; e += 256) {
+    int k = e / TR, li = e % TR;
     int64_t l = l0 + li;
     tile[li][k] = (l < m && k < N) ? HEc[l + (int64_t)k * m] : 0.0;
   }
   __syncthreads();
-  if (threadIdx.x < 64) {
+  if (threadIdx.x < TR) {
     int li = threadIdx.x;
     int64_t l = l0 + li;
     double s = 0;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_obs_anom(int64_t m, int N, const double
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * NP; e += 256) {
+  for (int e = threadIdx.x; e < TR * NP; e += 256) {
     int li = e / NP, k = e % NP;
     int64_t l = l0 + li;
     if (l < m) HS[l * NP + k] = (k < N) ? (tile[li][k] - ybar[li]) / scaling : 0.0;
@@ -608,7 +608,8 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 
 template <int NP>
 static void prep_loc_t(const PrepLocArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_prep_loc<NP>, dim3(nblk(A.m, 64)), dim3(256), 0, st, A.m, A.N, A.HSf,
+  constexpr int TR = NP > 64 ? 32 : 64;
+  hipLaunchKernelGGL((k_prep_loc<NP, TR>), dim3(nblk(A.m, TR)), dim3(256), 0, st, A.m, A.N, A.HSf,
                      A.ldHSf, A.Hxf, A.yo, A.rmse2, A.excl, A.HS, A.innov, A.rho, A.err);
 }
 
@@ -618,6 +619,7 @@ void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st) {
     case 16: prep_loc_t<16>(A, st); break;
     case 32: prep_loc_t<32>(A, st); break;
     case 64: prep_loc_t<64>(A, st); break;
+    case 128: prep_loc_t<128>(A, st); break;
   }
 }
 
@@ -628,7 +630,8 @@ static void prep_ens_t(const PrepEnsArgs& A, hipStream_t st) {
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs, dim3(nblk(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E, A.ldE, A.Hshift, A.HEc, A.err);
-  hipLaunchKernelGGL(k_obs_anom<NP>, dim3(nblk(A.m, 64)), dim3(256), 0, st, A.m, A.N, A.HEc, A.yo,
+  constexpr int TR = NP > 64 ? 32 : 64;
+  hipLaunchKernelGGL((k_obs_anom<NP, TR>), dim3(nblk(A.m, TR)), dim3(256), 0, st, A.m, A.N, A.HEc, A.yo,
                      A.rmse2, A.excl, A.scaling, A.HS, A.innov, A.rho, A.err);
 }
 
@@ -637,6 +640,7 @@ void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st) {
     case 16: prep_ens_t<16>(A, st); break;
     case 32: prep_ens_t<32>(A, st); break;
     case 64: prep_ens_t<64>(A, st); break;
+    case 128: prep_ens_t<128>(A, st); break;
   }
 }
 
@@ -670,6 +674,7 @@ void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
     case 16: chunk_t<16>(A, st, tm); break;
     case 32: chunk_t<32>(A, st, tm); break;
     case 64: chunk_t<64>(A, st, tm); break;
+    case 128: launch_chunk128(A, st, tm); break;
   }
 }
 

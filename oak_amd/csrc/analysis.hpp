@@ -93,5 +93,6 @@ struct ChunkArgs {
 void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
 void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 
 }  // namespace oakg

@@ -121,6 +121,7 @@ int padded_n(int N) {
   if (N <= 16) return 16;
   if (N <= 32) return 32;
   if (N <= 64) return 64;
+  if (N <= 128) return 128;
   return 0;
 }
 
@@ -331,7 +332,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   if (N < 1 || N > OAKG_MAX_N) arg_fail(c, "N out of range");
   const int NP = padded_n(N);
   if (!NP) {
-    c->err = "N above 64 is not supported by this build yet";
+    c->err = "N above 128 is not supported by this build";
     return OAKG_EUNSUP;
   }
   if (!csr) validate_zones_host(c, zones, 0, true);
@@ -598,7 +599,7 @@ int oakg_ensemble_analysis(oakg_ctx* c, const oakg_zones* zones, const oakg_obs*
     if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
     const int NP = padded_n(N);
     if (!NP) {
-      c->err = "N above 64 is not supported by this build yet";
+      c->err = "N above 128 is not supported by this build";
       return OAKG_EUNSUP;
     }
     validate_zones_host(c, zones, 0, true);

@@ -176,7 +176,14 @@ def test_ensemble_analysis_cfg0_vs_oracle(oak, cfg0):
     assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64
 
 
-@pytest.mark.parametrize("N", [5, 12, 24, 40, 64])
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_lapack():
+    O.load_lapack()
+    yield
+    O.set_backend(0)
+
+
+@pytest.mark.parametrize("N", [5, 12, 24, 40, 64, 100, 128])
 def test_ensemble_sizes_vs_oracle(oak, N):
     from oak_amd import synth
     _, A = oak

@@ -121,6 +121,14 @@ const char* oakg_last_error(const oakg_ctx* ctx);
 int oakg_set_stream(oakg_ctx* ctx, void* stream);
 /* 1 = record per-phase device times into oakg_stats (adds events). */
 int oakg_set_timing(oakg_ctx* ctx, int on);
+/* Eigensolver of the N x N ensemble-space matrix (OAK calls LAPACK dsyev,
+   matoper_inc.F90:986-1005):
+     OAKG_EIG_QL      Householder tridiagonalisation + implicit QL (default;
+                      zones where QL fails fall back to Jacobi)
+     OAKG_EIG_JACOBI  one-sided Jacobi for every zone                        */
+#define OAKG_EIG_JACOBI 0
+#define OAKG_EIG_QL 1
+int oakg_set_eigensolver(oakg_ctx* ctx, int method);
 int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
 /* Per-zone local observation counts p_z of the last call (host array). */
 int oakg_get_counts(oakg_ctx* ctx, int64_t* counts, int64_t nzones);

@@ -16,11 +16,12 @@ OAKG_OK, OAKG_EARG, OAKG_ENAN, OAKG_EHIP, OAKG_ENOMEM, OAKG_EUNSUP = range(6)
 OAKG_FP64, OAKG_MIXED = 0, 1
 OAKG_DEVICE_PTRS = 1
 OAKG_MAX_N = 128
+OAKG_EIG_JACOBI, OAKG_EIG_QL = 0, 1
 
 # Names every build of the library exports (checked by the CPU test suite).
 EXPORTS = [
     "oakg_version", "oakg_create", "oakg_destroy", "oakg_last_error", "oakg_set_stream",
-    "oakg_set_timing", "oakg_get_stats", "oakg_get_counts", "oakg_local_analysis",
+    "oakg_set_timing", "oakg_set_eigensolver", "oakg_get_stats", "oakg_get_counts", "oakg_local_analysis",
     "oakg_local_analysis_csr", "oakg_select_observations", "oakg_ensemble_analysis",
     "oakg_ensemble_analysis_f32", "oakg_coo_to_csr", "oakg_partition",
 ]
@@ -96,6 +97,7 @@ def load():
     L.oakg_destroy.argtypes = [_vp]
     L.oakg_set_stream.argtypes = [_vp, _vp]
     L.oakg_set_timing.argtypes = [_vp, C.c_int]
+    L.oakg_set_eigensolver.argtypes = [_vp, C.c_int]
     L.oakg_get_stats.argtypes = [_vp, C.POINTER(Stats)]
     L.oakg_get_counts.argtypes = [_vp, _vp, C.c_int64]
     L.oakg_local_analysis.argtypes = [
@@ -154,6 +156,10 @@ class Context:
 
     def set_timing(self, on=True):
         self.check(load().oakg_set_timing(self._h, 1 if on else 0))
+
+    def set_eigensolver(self, method):
+        """OAKG_EIG_QL (default) or OAKG_EIG_JACOBI."""
+        self.check(load().oakg_set_eigensolver(self._h, int(method)))
 
     def set_stream(self, stream_handle):
         self.check(load().oakg_set_stream(self._h, stream_handle))

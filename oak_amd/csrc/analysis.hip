@@ -305,7 +305,8 @@ __global__ __launch_bounds__(256) void k_eig(int64_t z0, int nzc, int N,
                                              double* __restrict__ ampl,
                                              int32_t* __restrict__ sweeps_out,
                                              unsigned* __restrict__ err, double tol,
-                                             int max_sweeps) {
+                                             int max_sweeps,
+                                             const int32_t* __restrict__ only) {
   constexpr int GPW = 64 / NP;
   constexpr int H = NP / 2;
   __shared__ EigLds<NP> lds[4];
@@ -313,7 +314,8 @@ __global__ __launch_bounds__(256) void k_eig(int64_t z0, int nzc, int N,
   EigLds<NP>& L = lds[wave];
   const int grp = lane / NP, r = lane % NP, gb = grp * NP;
   const int zl = (blockIdx.x * 4 + wave) * GPW + grp;
-  const bool valid = zl < nzc;
+  // with `only`, zones whose flag has no bit in 3 are left untouched
+  const bool valid = zl < nzc && (!only || (only[zl] & 3));
   const int64_t z = z0 + (valid ? zl : 0);
   const bool active = valid && (sel_ptr[z + 1] > sel_ptr[z]);
   const double* M = mat + (size_t)(valid ? zl : 0) * NP * NP;
@@ -651,9 +653,17 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   hipLaunchKernelGGL(k_gram<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.sel_idx,
                      A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
   if (tm) tm->mark(PH_EIG);
-  hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.nzc, A.N,
-                     A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
-                     A.max_sweeps);
+  if (A.eig_method == 1) {
+    launch_tql_pipeline(NP, A, st);
+    // zones the QL could not finish fall back to Jacobi
+    hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.nzc,
+                       A.N, A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
+                       A.max_sweeps, (const int32_t*)A.tql_status);
+  } else {
+    hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.nzc,
+                       A.N, A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
+                       A.max_sweeps, (const int32_t*)nullptr);
+  }
   if (tm) tm->mark(PH_TMAT);
   hipLaunchKernelGGL(k_tmat<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat,
                      A.mat, A.vec);

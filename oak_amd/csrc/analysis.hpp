@@ -78,6 +78,12 @@ struct ChunkArgs {
   const double *rho, *innov, *HS;
   double *mat, *ymat, *vec, *ampl;
   int32_t* sweeps;
+  // eigensolver: 0 = one-sided Jacobi, 1 = Householder tridiagonalisation + implicit QL
+  int eig_method;
+  double *dvec, *evec;      // tridiagonal / eigenvalues, NP x nzc
+  void* rlog;               // QL rotation log, rlog_cap records of 16 B per zone
+  int64_t rlog_cap;
+  int32_t* tql_status;      // per zone: bit0 log overflow, bit1 no convergence, >>2 log length
   unsigned* err;
   double tol;
   int max_sweeps;
@@ -94,5 +100,6 @@ void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
 void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
 
 }  // namespace oakg

@@ -181,11 +181,13 @@ __global__ __launch_bounds__(256) void k_eig128(int64_t z0, int nzc, int N,
                                                 double* __restrict__ ampl,
                                                 int32_t* __restrict__ sweeps_out,
                                                 unsigned* __restrict__ err, double tol,
-                                                int max_sweeps) {
+                                                int max_sweeps,
+                                                const int32_t* __restrict__ only) {
   __shared__ Eig128Lds L;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (only && !(only[zl] & 3)) return;
   double* V = vec + (size_t)zl * 4 * P128;
   if (sel_ptr[z + 1] == sel_ptr[z]) {
     if (tid < P128) V[P128 + tid] = 0.0;
@@ -587,8 +589,16 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   hipLaunchKernelGGL(k_gram128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.sel_idx, A.sel_w,
                      A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
   if (tm) tm->mark(PH_EIG);
-  hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.nzc, A.N, A.sel_ptr, A.mat,
-                     A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps);
+  if (A.eig_method == 1) {
+    launch_tql_pipeline(128, A, st);
+    hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.nzc, A.N, A.sel_ptr,
+                       A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps,
+                       (const int32_t*)A.tql_status);
+  } else {
+    hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.nzc, A.N, A.sel_ptr,
+                       A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps,
+                       (const int32_t*)nullptr);
+  }
   if (tm) tm->mark(PH_TMAT);
   hipLaunchKernelGGL(k_tmat128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat, A.mat,
                      A.vec);

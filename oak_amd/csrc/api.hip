@@ -27,6 +27,7 @@ struct oakg_ctx {
   oakg::PhaseTimer timer;
   double jacobi_tol = 0;  // 0 = default for the precision
   int max_sweeps = 40;
+  int eig_method = 1;     // 1 = tridiagonal QL (default), 0 = one-sided Jacobi
   // last call, for the lazily copied per-zone counts
   int64_t last_nz = 0;
   const int64_t* last_sel_ptr = nullptr;
@@ -85,6 +86,14 @@ __global__ __launch_bounds__(1024) void k_call_stats(int64_t nz, const int64_t* 
 void fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+__global__ void k_fill_i32b(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_i32b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 void fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st) {
   if (n > 0)
@@ -285,15 +294,30 @@ void materialize_counts(oakg_ctx* c) {
   c->counts_ready = true;
 }
 
-// Runs the chunked zone loop (gram, eig, tmat, update) for all zones.
+// Runs the chunked zone loop (gram, eigensolve, tmat, update) for all zones.
 void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_all) {
-  const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 4 * NP);
-  int64_t zc = std::min<int64_t>(nz, std::max<int64_t>(1, (int64_t)((size_t)3 << 30) / per_zone));
+  const int64_t cap = (int64_t)3 * NP * NP / 2 + 4 * NP;
+  const bool ql = c->eig_method == 1;
+  const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
+                          (ql ? (size_t)cap * 16 + 4 : 0);
+  const size_t budget = (size_t)8 << 30;
+  int64_t zc = std::min<int64_t>(nz, std::max<int64_t>(1, (int64_t)(budget / per_zone)));
   zc = std::min<int64_t>(zc, 1 << 20);
   double* mat = (double*)c->ws.get(Workspace::MAT, sizeof(double) * (size_t)NP * NP * zc);
   double* ymat = (double*)c->ws.get(Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zc);
   double* vec = (double*)c->ws.get(Workspace::VEC, sizeof(double) * (size_t)4 * NP * zc);
   if (!mat || !ymat || !vec) check(c, hipErrorOutOfMemory, "workspace");
+  double *dvec = nullptr, *evec = nullptr;
+  void* rlog = nullptr;
+  int32_t* tst = nullptr;
+  if (ql) {
+    dvec = (double*)c->ws.get(Workspace::DVEC, sizeof(double) * (size_t)NP * zc);
+    evec = (double*)c->ws.get(Workspace::EVEC, sizeof(double) * (size_t)NP * zc);
+    rlog = c->ws.get(Workspace::RLOG, (size_t)16 * cap * zc);
+    tst = (int32_t*)c->ws.get(Workspace::TQL_STATUS, sizeof(int32_t) * zc);
+    if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
+    fill_i32(sweeps_all, nz, 0, c->stream);
+  }
   for (int64_t z0 = 0; z0 < nz; z0 += zc) {
     ChunkArgs A = base;
     A.z0 = z0;
@@ -302,6 +326,12 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     A.ymat = ymat;
     A.vec = vec;
     A.sweeps = sweeps_all + z0;
+    A.eig_method = ql ? 1 : 0;
+    A.dvec = dvec;
+    A.evec = evec;
+    A.rlog = rlog;
+    A.rlog_cap = cap;
+    A.tql_status = tst;
     launch_chunk(NP, A, c->stream, c->timing ? &c->timer : nullptr);
     check(c, hipGetLastError(), "chunk launch");
   }
@@ -519,6 +549,16 @@ int oakg_set_stream(oakg_ctx* c, void* stream) {
 int oakg_set_timing(oakg_ctx* c, int on) {
   if (!c) return OAKG_EARG;
   c->timing = on != 0;
+  return OAKG_OK;
+}
+
+int oakg_set_eigensolver(oakg_ctx* c, int method) {
+  if (!c) return OAKG_EARG;
+  if (method != OAKG_EIG_JACOBI && method != OAKG_EIG_QL) {
+    c->err = "oakg_set_eigensolver: unknown method";
+    return OAKG_EARG;
+  }
+  c->eig_method = method;
   return OAKG_OK;
 }
 

@@ -183,22 +183,33 @@ def _oracle_lapack():
     O.set_backend(0)
 
 
+@pytest.fixture(scope="module")
+def jacobi_ctx():
+    from oak_amd._lib import OAKG_EIG_JACOBI, Context
+    c = Context(0)
+    c.set_eigensolver(OAKG_EIG_JACOBI)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("eig", ["ql", "jacobi"])
 @pytest.mark.parametrize("N", [5, 12, 24, 40, 64, 100, 128])
-def test_ensemble_sizes_vs_oracle(oak, N):
+def test_ensemble_sizes_vs_oracle(oak, N, eig, request):
     from oak_amd import synth
     _, A = oak
+    ctx = request.getfixturevalue("jacobi_ctx") if eig == "jacobi" else None
     P = synth.column_ocean(12, 9, 3, N, 60, 4, seed_cfg=100 + N, hlen=3, vlen=2, L=2.0,
                            cutoff=5.0)
     H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
     for infl, maxc in ((1.0, np.inf), (1.07, 0.05)):
         Ea = A.ensemble_analysis(_geometry(P), H, P.Hshift, P.yo, P.rmse2, P.E, inflation=infl,
-                                 max_correction=maxc)
+                                 max_correction=maxc, ctx=ctx)
         geo = dict(zone_ptr=P.zone_ptr, zpos=P.zpos, L=P.corr_len, cutoff=P.max_len, metric=0,
                    loctype=1, ox=P.ox, oy=P.oy)
         Eo, _, _ = O.ensemble_analysis(geo, (P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
                                        inflation=infl, max_correction=maxc)
         xf = P.E.mean(axis=1)
-        assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64, (N, infl)
+        assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64, (N, infl, eig)
 
 
 @pytest.mark.parametrize("metric", [1, 2])

@@ -4,7 +4,13 @@ One step = one call of the analysis (oak_assim's ensemble path: H·E, anomalies,
 selection, per-zone S^T R^-1 S, eigensolve, transform, Ea) over the whole
 synthetic ensemble, with every input already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--config 1] [--precision fp64]
+    python bench.py [--steps K] [--warmup W] [--config 2] [--precision both|fp64|mixed]
+
+The default workload is config 2 (360x180x30x4+SSH grid, N=128, clustered
+observations): the mixed-precision analysis (FP32 state, FP32 MFMA
+contractions, FP64 eigensolve) is the headline ``value``; the FP64 analysis of
+the same inputs is timed beside it (``fp64``, ``mixed_speedup_vs_fp64``) and
+the mixed result is compared with it (``mixed_err_vs_fp64``).
 
 Prints one JSON line (rank 0).  The metric is analysed state elements per
 second: n / t_step, an element being one entry of xa with its N-member row of
@@ -118,13 +124,63 @@ def cpu_baseline(DP, target_s=15.0):
                       % (nzs, ne, "LAPACK dsyev (OpenBLAS)" if lapack else "Jacobi", t)}
 
 
+PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
+
+
+def run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea, label):
+    """Warmup + timed steps of one precision; returns (ms/step, phase ms, stats)."""
+    import torch
+    from oak_amd.assimilation import ensemble_analysis
+    for i in range(args.warmup):
+        ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
+        torch.cuda.synchronize()
+        print("# %s warmup %d done: %s" % (label, i, ctx.stats()), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    phase = {"gram": 0.0, "eig": 0.0, "tmat": 0.0, "update": 0.0, "select": 0.0, "prep": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
+        st = ctx.stats()  # phase times of this call (device events)
+        for k in phase:
+            phase[k] += st["ms_" + k]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    for k in phase:
+        phase[k] /= args.steps
+    print("# %s timed %d steps: %.3f ms/step %s" % (label, args.steps, ms, st), file=sys.stderr,
+          flush=True)
+    return ms, phase, st
+
+
+def roofline(F, phase, mixed, eig_name):
+    """Dominant phase of the step and its achieved rate against the MI355X peak
+    of the arithmetic it runs on (FP32 MFMA for the mixed contractions, FP64
+    otherwise)."""
+    work = {"gram": F["gram"], "eig": F["eig"], "tmat": F["trans"], "update": F["update"]}
+    kern = max(work, key=lambda k: phase[k])
+    names = {
+        "gram": ("k_gram_f32 (S^T R^-1 S, v_mfma_f32_16x16x4f32)" if mixed else
+                 "k_gram (S^T R^-1 S, v_mfma_f64_16x16x4f64)"),
+        "eig": eig_name,
+        "tmat": "k_tmat (T = Y Y^T Omega, v_mfma_f64_16x16x4f64)",
+        "update": ("k_update_f32 (Ea = xa + A' W, v_mfma_f32_16x16x4f32)" if mixed else
+                   "k_update (Ea = xa + A' W, v_mfma_f64_16x16x4f64)"),
+    }
+    peak = PEAK_FP32_MFMA if (mixed and kern in ("gram", "update")) else PEAK_FP64
+    ach = work[kern] / (phase[kern] * 1e-3)
+    return {"bound": "mfma", "kernel": names[kern], "achieved": ach / 1e12, "peak": peak / 1e12,
+            "unit": "TFLOP/s", "frac": ach / peak, "traffic": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed", "both"],
+                    help="default: both for config 2 (the headline: mixed timed against FP64), "
+                         "fp64 otherwise")
     ap.add_argument("--eig", default="ql", choices=["ql", "jacobi"],
                     help="eigensolver: tridiagonal QL (default) or one-sided Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
@@ -137,10 +193,11 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    prec = args.precision or ("both" if args.config == 2 else "fp64")
     import torch
     from oak_amd import Context, OAKG_FP64, OAKG_MIXED
     from oak_amd._lib import OAKG_EIG_JACOBI, OAKG_EIG_QL
-    from oak_amd.assimilation import ensemble_analysis
+    from oak_amd.errors import ensemble_errors
     device = torch.device("cuda:0")
     if args.calib_only:
         src = torch.ones(1 << 27, dtype=torch.float64, device=device)
@@ -154,62 +211,60 @@ def main():
     DP, desc = build_problem(args.config, device)
     P = DP.host
     g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
-    gen_s = time.time() - t0
-    print("# generated %s in %.1f s" % (desc, gen_s), file=sys.stderr, flush=True)
-    prec = OAKG_MIXED if args.precision == "mixed" else OAKG_FP64
-    ctx = Context(0, prec)
-    E = DP.E
-    n, N = E.shape
-    Ea = torch.empty((N, n), dtype=E.dtype, device=device).t()
-    ctx.set_timing(True)
-    ctx.set_eigensolver(OAKG_EIG_QL if args.eig == "ql" else OAKG_EIG_JACOBI)
-    for i in range(args.warmup):
-        ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
-        torch.cuda.synchronize()
-        print("# warmup %d done: %s" % (i, ctx.stats()), file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    phase = {"gram": 0.0, "eig": 0.0, "tmat": 0.0, "update": 0.0, "select": 0.0, "prep": 0.0}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
-        st = ctx.stats()  # phase times of this call (device events)
-        for k in phase:
-            phase[k] += st["ms_" + k]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    for k in phase:
-        phase[k] /= args.steps
-    counts = ctx.counts(P.nzones)
+    print("# generated %s in %.1f s" % (desc, time.time() - t0), file=sys.stderr, flush=True)
+    n, N = DP.E.shape
+    eig_name = ("k_tridiag+k_tql+k_tql_apply+k_eigvec_epi (Householder + implicit QL, FP64)"
+                if args.eig == "ql" else "k_eig (one-sided Jacobi, FP64)")
+    eig_id = OAKG_EIG_QL if args.eig == "ql" else OAKG_EIG_JACOBI
+    res = {}
+    Ea64 = None
+    if prec in ("fp64", "both"):
+        ctx = Context(0, OAKG_FP64)
+        ctx.set_timing(True)
+        ctx.set_eigensolver(eig_id)
+        E = DP.E.double() if DP.E.dtype != torch.float64 else DP.E
+        Ea64 = torch.empty((N, n), dtype=torch.float64, device=device).t()
+        res["fp64"] = run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea64, "fp64")
+        counts = ctx.counts(P.nzones)
+        ctx.close()
+    if prec in ("mixed", "both"):
+        ctx = Context(0, OAKG_MIXED)
+        ctx.set_timing(True)
+        ctx.set_eigensolver(eig_id)
+        E32 = torch.empty((N, n), dtype=torch.float32, device=device).t()
+        E32.copy_(DP.E)
+        Ea32 = torch.empty((N, n), dtype=torch.float32, device=device).t()
+        res["mixed"] = run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E32, Ea32, "mixed")
+        counts = ctx.counts(P.nzones)
+        ctx.close()
+        del E32
+    head = "mixed" if "mixed" in res else "fp64"
+    ms, phase, _ = res[head]
     zs = np.diff(P.zone_ptr)
     F = flops_model(N, counts, zs)
-    # dominant kernel
-    kern = max(("gram", "eig"), key=lambda k: phase[k])
-    if kern == "gram":
-        ach = F["gram"] / (phase["gram"] * 1e-3)
-        kname = "k_gram (S^T R^-1 S, v_mfma_f64_16x16x4f64)"
-    else:
-        ach = F["eig"] / (phase["eig"] * 1e-3)
-        kname = ("k_tridiag+k_tql+k_tql_apply+k_eigvec_epi (Householder + implicit QL, FP64)"
-                 if args.eig == "ql" else "k_eig (one-sided Jacobi, FP64)")
-    roof = {"bound": "mfma", "kernel": kname, "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
-            "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None}
     out = {
         "metric": "analysed state elems/s",
         "value": n / (ms * 1e-3),
         "unit": "analysed state elems/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == OAKG_FP64 else "f32/bf16+f64",
-        "data": "synthetic (seeded GRF ensembles and observations)",
+        "dtype": "f64" if head == "fp64" else "f32 state, f32 MFMA, f64 eigensolve",
+        "data": "synthetic (seeded GRF ensembles and observations, random-init, no datasets)",
         "config": {"workload": "config%d: %s, %s, %s eigensolver" % (
-                       args.config, desc, args.precision, args.eig),
+                       args.config, desc, head, args.eig),
                    "n": int(n), "N": int(N), "m": int(P.m), "zones": int(P.nzones),
-                   "mean_p": float(counts.mean()), "max_p": int(counts.max()),
+                   "mean_p": float(counts.mean()), "median_p": float(np.median(counts)),
+                   "max_p": int(counts.max()),
                    "phase_ms": {k: round(v, 3) for k, v in phase.items()}},
-        "roofline": roof,
+        "roofline": roofline(F, phase, head == "mixed", eig_name),
     }
-    print("# timed %d steps: %.3f ms/step" % (args.steps, ms), file=sys.stderr, flush=True)
+    if "mixed" in res and "fp64" in res:
+        ms64, ph64, _ = res["fp64"]
+        out["fp64"] = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64,
+                       "phase_ms": {k: round(v, 3) for k, v in ph64.items()}}
+        out["mixed_speedup_vs_fp64"] = ms64 / ms
+        e = ensemble_errors(Ea32, Ea64, DP.E)
+        out["mixed_err_vs_fp64"] = dict(e, bound=1e-4)
     if args.calib:
         # a plain copy of known size: 1 GiB read + 1 GiB written
         src = torch.ones(1 << 27, dtype=torch.float64, device=device)

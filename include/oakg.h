@@ -105,7 +105,7 @@ typedef struct {
   int64_t obs_selected;    /* sum over zones of the local observation count p_z  */
   int64_t pmax;            /* largest p_z                                        */
   int32_t sweeps_max;      /* most Jacobi sweeps used by one zone                */
-  int32_t reserved;
+  int32_t eig_fallbacks;   /* zones the QL eigensolver left to Jacobi            */
   double ms_total;         /* device time of the whole call                      */
   /* device time per phase, recorded only with oakg_set_timing(ctx, 1):
      prep (H·E, anomalies), select, gram (S^T R^-1 S), eig (eigensolve),

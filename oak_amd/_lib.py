@@ -60,12 +60,12 @@ class Csr(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("zones", C.c_int64), ("zones_analysed", C.c_int64),
                 ("obs_selected", C.c_int64), ("pmax", C.c_int64), ("sweeps_max", C.c_int32),
-                ("reserved", C.c_int32), ("ms_total", C.c_double), ("ms_prep", C.c_double),
+                ("eig_fallbacks", C.c_int32), ("ms_total", C.c_double), ("ms_prep", C.c_double),
                 ("ms_select", C.c_double), ("ms_gram", C.c_double), ("ms_eig", C.c_double),
                 ("ms_tmat", C.c_double), ("ms_update", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None

@@ -198,7 +198,10 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
     """oak_assim for the local scheme: Ef (n x N, zone-permuted) -> Ea.
 
     H = (row_ptr, col, val) in CSR by observation (see coo_to_csr).
-    With torch device tensors, ``out`` may give the preallocated Ea."""
+    With torch device tensors, ``out`` may give the preallocated Ea.
+    A float32 ensemble (numpy or torch) runs the mixed-precision path
+    (oakg_ensemble_analysis_f32: FP32 state and MFMA contractions, FP64
+    eigensolve); float64 runs the FP64 path."""
     ctx = ctx or default_context()
     L = _lib.load()
     dev = _is_dev(Ef)
@@ -214,22 +217,30 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
         ldA = _check_dev_matrix(Ea, n, N, "Ea")
         z, o = g.structs(yo, rmse2, excluded)
         h = Csr(m, n, len(va), rp.data_ptr(), co.data_ptr(), va.data_ptr())
-        fn = L.oakg_ensemble_analysis if Ef.dtype == torch.float64 else L.oakg_ensemble_analysis_f32
+        if Ef.dtype == torch.float64:
+            fn = L.oakg_ensemble_analysis
+        elif Ef.dtype == torch.float32:
+            fn = L.oakg_ensemble_analysis_f32
+            Hshift = Hshift.to(torch.float32).contiguous()
+        else:
+            raise TypeError("Ef must be float64 or float32")
         rc = fn(ctx.handle, C.byref(z), C.byref(o), C.byref(h), Hshift.data_ptr(), N,
                 Ef.data_ptr(), ldE, Ea.data_ptr(), ldA, float(inflation), maxc,
                 _lib.OAKG_DEVICE_PTRS)
         ctx.check(rc)
         return Ea
-    Ef = np.asfortranarray(Ef, dtype=np.float64)
-    yo = _host_vec(yo); rmse2 = _host_vec(rmse2); Hshift = _host_vec(Hshift)
+    mixed = np.asarray(Ef).dtype == np.float32
+    sdt = np.float32 if mixed else np.float64
+    Ef = np.asfortranarray(Ef, dtype=sdt)
+    yo = _host_vec(yo); rmse2 = _host_vec(rmse2); Hshift = _host_vec(Hshift, sdt)
     ex = _host_vec(excluded, np.uint8)
     rp = np.ascontiguousarray(rp, np.int64); co = np.ascontiguousarray(co, np.int32)
     va = np.ascontiguousarray(va, np.float64)
     z, o = g.structs(yo, rmse2, ex)
     h = Csr(m, n, len(va), rp.ctypes.data, co.ctypes.data, va.ctypes.data)
-    Ea = np.zeros((n, N), order="F")
-    rc = L.oakg_ensemble_analysis(ctx.handle, C.byref(z), C.byref(o), C.byref(h),
-                                  Hshift.ctypes.data, N, Ef.ctypes.data, n, Ea.ctypes.data, n,
-                                  float(inflation), maxc, 0)
+    Ea = np.zeros((n, N), order="F", dtype=sdt)
+    fn = L.oakg_ensemble_analysis_f32 if mixed else L.oakg_ensemble_analysis
+    rc = fn(ctx.handle, C.byref(z), C.byref(o), C.byref(h), Hshift.ctypes.data, N,
+            Ef.ctypes.data, n, Ea.ctypes.data, n, float(inflation), maxc, 0)
     ctx.check(rc)
     return Ea

@@ -137,15 +137,17 @@ __global__ __launch_bounds__(256) void k_obs_anom(int64_t m, int N, const double
 // Gram matrix per zone (rrsqrt.F90:135 and the S^T R^-1 d factor of :142).
 // One workgroup (4 waves) per zone; the waves split the local observations in
 // steps of 4 and each wave accumulates every lower 16x16 tile; the partial
-// tiles are summed across waves in a fixed order (deterministic).
+// tiles are summed across waves in a fixed order (deterministic).  TS is the
+// stored type of S (double, or float for the mixed path: products of FP32
+// values are exact in FP64, so the FP64 accumulation adds no FP32 error).
 // ---------------------------------------------------------------------------
-template <int NP>
+template <int NP, typename TS>
 __global__ __launch_bounds__(256) void k_gram(int64_t z0, const int64_t* __restrict__ sel_ptr,
                                               const int32_t* __restrict__ sel_idx,
                                               const double* __restrict__ sel_w, int64_t m,
                                               const double* __restrict__ rho,
                                               const double* __restrict__ innov,
-                                              const double* __restrict__ HS,
+                                              const TS* __restrict__ HS,
                                               double* __restrict__ mat, double* __restrict__ vec,
                                               unsigned* __restrict__ err) {
   constexpr int NB = NP / 16;
@@ -186,9 +188,9 @@ __global__ __launch_bounds__(256) void k_gram(int64_t z0, const int64_t* __restr
       }
     }
     double bv[NB];
-    const double* row = HS + (size_t)l * NP + jl;
+    const TS* row = HS + (size_t)l * NP + jl;
 #pragma unroll
-    for (int J = 0; J < NB; ++J) bv[J] = ok ? row[16 * J] : 0.0;
+    for (int J = 0; J < NB; ++J) bv[J] = ok ? (double)row[16 * J] : 0.0;
     int t = 0;
 #pragma unroll
     for (int I = 0; I < NB; ++I) {
@@ -638,6 +640,10 @@ static void prep_ens_t(const PrepEnsArgs& A, hipStream_t st) {
 }
 
 void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st) {
+  if (A.E32) {
+    launch_prep_ens_f32(NP, A, st);
+    return;
+  }
   switch (NP) {
     case 16: prep_ens_t<16>(A, st); break;
     case 32: prep_ens_t<32>(A, st); break;
@@ -650,8 +656,12 @@ template <int NP>
 static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   constexpr int GPW = 64 / NP;
   if (tm) tm->mark(PH_GRAM);
-  hipLaunchKernelGGL(k_gram<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.sel_idx,
-                     A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
+  if (A.prec == 1)
+    hipLaunchKernelGGL((k_gram<NP, float>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
+  else
+    hipLaunchKernelGGL((k_gram<NP, double>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
   if (tm) tm->mark(PH_EIG);
   if (A.eig_method == 1) {
     launch_tql_pipeline(NP, A, st);
@@ -668,7 +678,9 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   hipLaunchKernelGGL(k_tmat<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat,
                      A.mat, A.vec);
   if (tm) tm->mark(PH_UPDATE);
-  if (A.mode == 0)
+  if (A.prec == 1)
+    launch_update_f32(NP, A, st);
+  else if (A.mode == 0)
     hipLaunchKernelGGL((k_update<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr,
                        A.mat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl,
                        A.maxcorr);

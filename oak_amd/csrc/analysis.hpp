@@ -65,6 +65,10 @@ struct PrepEnsArgs {
   double scaling;
   double *xbar, *HEc, *HS, *innov, *rho;
   unsigned* err;
+  // mixed precision (FP32 state): E32, Hshift32 in, HS32 out
+  const float* E32;
+  const float* Hshift32;
+  float* HS32;
 };
 
 struct ChunkArgs {
@@ -94,6 +98,11 @@ struct ChunkArgs {
   double* Out;  // Sa (mode 0) or Ea (mode 1)
   int64_t ldO;
   double scaling, infl, maxcorr;
+  // mixed precision: 1 = FP32 S (HS32), FP32 members (S32) and FP32 Out32
+  int prec;
+  const float* HS32;
+  const float* S32;
+  float* Out32;
 };
 
 void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
@@ -101,5 +110,7 @@ void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
+void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);
+void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st);
 
 }  // namespace oakg

@@ -2,7 +2,8 @@
 // the same steps as analysis.hip (rrsqrt.F90:100-190), with kernels shaped
 // for a 128 x 128 ensemble-space matrix.
 //
-//   k_gram128  M = I + S^T diag(r) S on v_mfma_f64_16x16x4f64: 36 lower tiles,
+//   k_gram128  M = I + S^T diag(r) S on v_mfma_f64_16x16x4f64 (S stored FP64, or
+//              FP32 for the mixed path): 36 lower tiles,
 //              wave w owns tile rows {w, 7-w} (9 tiles each, no cross-wave sum)
 //   k_eig128   block one-sided Jacobi: each wave holds two 16-column blocks of
 //              all 128 rows (2 rows per lane, 64 doubles in registers); a sweep
@@ -21,12 +22,12 @@ constexpr int P128 = 128;
 // ---------------------------------------------------------------------------
 // Gram matrix, NP = 128
 // ---------------------------------------------------------------------------
-template <int W>
+template <int W, typename TS>
 __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_t* __restrict__ sel_idx,
                                              const double* __restrict__ sel_w, int64_t m,
                                              const double* __restrict__ rho,
                                              const double* __restrict__ innov,
-                                             const double* __restrict__ HS, double* __restrict__ M,
+                                             const TS* __restrict__ HS, double* __restrict__ M,
                                              double* __restrict__ bout, unsigned* __restrict__ err,
                                              int lane) {
   constexpr int IA = W, IB = 7 - W;
@@ -58,9 +59,9 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
       }
     }
     double bv[8];
-    const double* row = HS + (size_t)l * P128 + jl;
+    const TS* row = HS + (size_t)l * P128 + jl;
 #pragma unroll
-    for (int J = 0; J < 8; ++J) bv[J] = ok ? row[16 * J] : 0.0;
+    for (int J = 0; J < 8; ++J) bv[J] = ok ? (double)row[16 * J] : 0.0;
     const double aA = r * bv[IA], aB = r * bv[IB];
 #pragma unroll
     for (int J = 0; J <= IA; ++J) accA[J] = mfma_f64(aA, bv[J], accA[J]);
@@ -100,12 +101,13 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
   }
 }
 
+template <typename TS>
 __global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int64_t* __restrict__ sel_ptr,
                                                  const int32_t* __restrict__ sel_idx,
                                                  const double* __restrict__ sel_w, int64_t m,
                                                  const double* __restrict__ rho,
                                                  const double* __restrict__ innov,
-                                                 const double* __restrict__ HS,
+                                                 const TS* __restrict__ HS,
                                                  double* __restrict__ mat, double* __restrict__ vec,
                                                  unsigned* __restrict__ err) {
   const int zl = blockIdx.x;
@@ -121,10 +123,10 @@ __global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int64_t* __re
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   switch (wave) {
-    case 0: gram128_wave<0>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
-    case 1: gram128_wave<1>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
-    case 2: gram128_wave<2>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
-    default: gram128_wave<3>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
+    case 0: gram128_wave<0, TS>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
+    case 1: gram128_wave<1, TS>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
+    case 2: gram128_wave<2, TS>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
+    default: gram128_wave<3, TS>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane); break;
   }
 }
 
@@ -586,8 +588,12 @@ __global__ __launch_bounds__(256) void k_update128(int64_t z0, int N, const int6
 
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_GRAM);
-  hipLaunchKernelGGL(k_gram128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.sel_idx, A.sel_w,
-                     A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
+  if (A.prec == 1)
+    hipLaunchKernelGGL(k_gram128<float>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
+  else
+    hipLaunchKernelGGL(k_gram128<double>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
   if (tm) tm->mark(PH_EIG);
   if (A.eig_method == 1) {
     launch_tql_pipeline(128, A, st);
@@ -603,7 +609,9 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   hipLaunchKernelGGL(k_tmat128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat, A.mat,
                      A.vec);
   if (tm) tm->mark(PH_UPDATE);
-  if (A.mode == 0)
+  if (A.prec == 1)
+    launch_update_f32(128, A, st);
+  else if (A.mode == 0)
     hipLaunchKernelGGL(k_update128<0>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr, A.mat,
                        A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl, A.maxcorr);
   else

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/oakg.h"
@@ -50,7 +51,8 @@ __global__ void k_fill_f64(double* p, int64_t n, double v) {
   if (i < n) p[i] = v;
 }
 // Per-call statistics reduced on the device: [0] error bits, [1] zones with
-// p > 0, [2] sum p, [3] max p, [4] max sweeps.  One block.
+// p > 0, [2] sum p, [3] max p, [4] max sweeps, [5] zones the QL eigensolver
+// handed to the Jacobi fallback.  One block.
 __global__ __launch_bounds__(1024) void k_call_stats(int64_t nz, const int64_t* __restrict__ sel_ptr,
                                                      const int32_t* __restrict__ sweeps,
                                                      const unsigned* __restrict__ err,
@@ -75,7 +77,8 @@ __global__ __launch_bounds__(1024) void k_call_stats(int64_t nz, const int64_t* 
     __syncthreads();
   }
   if (t == 0) {
-    out[0] = (int64_t)*err;
+    out[0] = (int64_t)err[0];
+    out[5] = (int64_t)err[1];
     out[1] = sh[0][0];
     out[2] = nz > 0 ? sel_ptr[nz] - sel_ptr[0] : 0;
     out[3] = sh[1][0];
@@ -147,18 +150,19 @@ const T* stage(oakg_ctx* c, Workspace::Slot s, const T* p, int64_t count, bool d
 
 // Stages a column-major rows x cols block with leading dimension ld into a
 // dense (ld = rows) device copy.
-const double* stage2d(oakg_ctx* c, Workspace::Slot s, const double* p, int64_t rows, int cols,
-                      int64_t ld, bool dev, int64_t* ld_out) {
+template <typename T>
+const T* stage2d(oakg_ctx* c, Workspace::Slot s, const T* p, int64_t rows, int cols, int64_t ld,
+                 bool dev, int64_t* ld_out) {
   if (dev) {
     *ld_out = ld;
     return p;
   }
-  double* d = (double*)c->ws.get(s, sizeof(double) * std::max<int64_t>(rows, 1) * cols);
+  T* d = (T*)c->ws.get(s, sizeof(T) * std::max<int64_t>(rows, 1) * cols);
   if (!d) check(c, hipErrorOutOfMemory, "staging");
   if (rows > 0)
     check(c,
-          hipMemcpy2DAsync(d, sizeof(double) * rows, p, sizeof(double) * ld, sizeof(double) * rows,
-                           cols, hipMemcpyHostToDevice, c->stream),
+          hipMemcpy2DAsync(d, sizeof(T) * rows, p, sizeof(T) * ld, sizeof(T) * rows, cols,
+                           hipMemcpyHostToDevice, c->stream),
           "H2D 2D");
   *ld_out = rows;
   return d;
@@ -229,10 +233,10 @@ void begin_call(oakg_ctx* c) {
   c->err.clear();
   check(c, hipSetDevice(c->device), "hipSetDevice");
   if (!c->d_err) {
-    check(c, hipMalloc(&c->d_err, sizeof(unsigned)), "hipMalloc");
+    check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
     check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
   }
-  fill_u32(c->d_err, 1, 0u, c->stream);
+  fill_u32(c->d_err, 2, 0u, c->stream);  // [0] error bits, [1] eigensolver fallbacks
   memset(&c->stats, 0, sizeof c->stats);
   c->counts_ready = true;
   c->counts.clear();
@@ -243,7 +247,7 @@ void begin_call(oakg_ctx* c) {
 // the error word in one copy, synchronizes, and turns device errors into
 // return codes.  sel_ptr may be NULL (no zone loop in the call).
 void finish_call(oakg_ctx* c, int64_t nz, const int64_t* sel_ptr, const int32_t* sweeps) {
-  int64_t h[5] = {0, 0, 0, 0, 0};
+  int64_t h[6] = {0, 0, 0, 0, 0, 0};
   hipLaunchKernelGGL(k_call_stats, dim3(1), dim3(1024), 0, c->stream, sel_ptr ? nz : 0,
                      sel_ptr, sweeps, c->d_err, c->d_stats);
   check(c, hipGetLastError(), "stats launch");
@@ -253,6 +257,7 @@ void finish_call(oakg_ctx* c, int64_t nz, const int64_t* sel_ptr, const int32_t*
   c->stats.obs_selected = h[2];
   c->stats.pmax = h[3];
   c->stats.sweeps_max = (int32_t)h[4];
+  c->stats.eig_fallbacks = (int32_t)h[5];
   const unsigned herr = (unsigned)h[0];
   if (herr & OAKG_DERR_RMSE) arg_fail(c, "observation error variance rmse^2 <= 0 or not finite");
   if (herr & OAKG_DERR_IDX) arg_fail(c, "index out of range in H or in the selection");
@@ -296,7 +301,10 @@ void materialize_counts(oakg_ctx* c) {
 
 // Runs the chunked zone loop (gram, eigensolve, tmat, update) for all zones.
 void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_all) {
-  const int64_t cap = (int64_t)3 * NP * NP / 2 + 4 * NP;
+  // QL rotation log: records of 16 B per zone, a multiple of 4 (the log is
+  // interleaved in groups of 4 records over the 64 zones of a QL wave)
+  // (about 1.1 N^2 records for typical zones; 2 N^2 + 8 N leaves room for slow ones)
+  const int64_t cap = (((int64_t)2 * NP * NP + 8 * NP) + 3) & ~(int64_t)3;
   const bool ql = c->eig_method == 1;
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
@@ -313,7 +321,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   if (ql) {
     dvec = (double*)c->ws.get(Workspace::DVEC, sizeof(double) * (size_t)NP * zc);
     evec = (double*)c->ws.get(Workspace::EVEC, sizeof(double) * (size_t)NP * zc);
-    rlog = c->ws.get(Workspace::RLOG, (size_t)16 * cap * zc);
+    rlog = c->ws.get(Workspace::RLOG, (size_t)16 * cap * ((zc + 63) & ~(int64_t)63));
     tst = (int32_t*)c->ws.get(Workspace::TQL_STATUS, sizeof(int32_t) * zc);
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
     fill_i32(sweeps_all, nz, 0, c->stream);
@@ -506,6 +514,152 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   return OAKG_OK;
 }
 
+// Ensemble analysis of oak_assim (assimilation.F90:3083-3134, 3235, 3299-3320):
+// T = double runs the FP64 path, T = float the mixed-precision path.
+template <typename T>
+int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, const oakg_csr* H,
+                  const T* Hshift, int32_t N, const T* Ef, int64_t ldEf, T* Ea, int64_t ldEa,
+                  double inflation, double max_correction, uint32_t flags) {
+  constexpr bool mixed = std::is_same<T, float>::value;
+  return guarded(c, [&] {
+    const bool dev = flags & OAKG_DEVICE_PTRS;
+    if (!zones || !obs || !H || !Ef || !Ea) arg_fail(c, "zones, obs, H, Ef and Ea are required");
+    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+    const int NP = padded_n(N);
+    if (!NP) {
+      c->err = "N above 128 is not supported by this build";
+      return OAKG_EUNSUP;
+    }
+    validate_zones_host(c, zones, 0, true);
+    validate_obs(c, obs, true, zones->loctype);
+    const int64_t nz = zones->nzones, m = obs->m;
+    if (H->m != m) arg_fail(c, "H->m must equal obs->m");
+    if (m > 0 && (!H->row_ptr || (H->nnz > 0 && (!H->col || !H->val)) || !Hshift))
+      arg_fail(c, "H arrays and Hshift are required");
+    int64_t n = 0;
+    if (!dev) {
+      validate_zone_ptr_host(c, nz, zones->zone_ptr);
+      n = zones->zone_ptr[nz];
+      if (H->row_ptr[0] != 0 || H->row_ptr[m] != H->nnz) arg_fail(c, "H->row_ptr inconsistent");
+      for (int64_t e = 0; e < H->nnz; ++e)
+        if (H->col[e] < 0 || H->col[e] >= n) arg_fail(c, "H column out of range");
+    } else if (nz > 0) {
+      check(c, hipMemcpy(&n, zones->zone_ptr + nz, sizeof(int64_t), hipMemcpyDeviceToHost), "D2H n");
+    }
+    if (H->n != n) arg_fail(c, "H->n must equal the state size");
+    if (ldEf < n || ldEa < n) arg_fail(c, "leading dimension too small");
+    if (!(inflation > 0) || !(max_correction > 0)) arg_fail(c, "inflation and max_correction must be > 0");
+    begin_call(c);
+    if (c->timing) c->timer.mark(PH_PREP);
+    hipEvent_t e0, e1;
+    check(c, hipEventCreate(&e0), "event");
+    check(c, hipEventCreate(&e1), "event");
+    check(c, hipEventRecord(e0, c->stream), "event");
+    Geo g = stage_geo(c, zones, obs, dev, true);
+    int64_t ld_e = 0;
+    const T* dE = stage2d(c, Workspace::IN_S, Ef, n, N, ldEf, dev, &ld_e);
+    const int64_t* drp = stage(c, Workspace::H_RP, H->row_ptr, m + 1, dev);
+    const int32_t* dcol = stage(c, Workspace::H_COL, H->col, H->nnz, dev);
+    const double* dval = stage(c, Workspace::H_VAL, H->val, H->nnz, dev);
+    const T* dHs = stage(c, Workspace::H_SHIFT, Hshift, m, dev);
+    double* xbar = (double*)c->ws.get(Workspace::XBAR, sizeof(double) * std::max<int64_t>(n, 1));
+    double* HEc = (double*)c->ws.get(Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
+    double* HS = nullptr;
+    float* HS32 = nullptr;
+    if (mixed)
+      HS32 = (float*)c->ws.get(Workspace::HS32, sizeof(float) * std::max<int64_t>(m, 1) * NP);
+    else
+      HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+    double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+    double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+    if (!xbar || !HEc || !(mixed ? (void*)HS32 : (void*)HS) || !innov || !rho)
+      check(c, hipErrorOutOfMemory, "workspace");
+    const double scaling = sqrt((double)N - 1.0);
+    PrepEnsArgs P{};
+    P.n = n;
+    P.m = m;
+    P.N = N;
+    P.ldE = ld_e;
+    P.row_ptr = drp;
+    P.col = dcol;
+    P.val = dval;
+    P.yo = g.od.yo;
+    P.rmse2 = g.od.rmse2;
+    P.excl = g.od.excluded;
+    P.scaling = scaling;
+    P.xbar = xbar;
+    P.HEc = HEc;
+    P.HS = HS;
+    P.HS32 = HS32;
+    P.innov = innov;
+    P.rho = rho;
+    P.err = c->d_err;
+    if (mixed) {
+      P.E32 = (const float*)dE;
+      P.Hshift32 = (const float*)dHs;
+    } else {
+      P.E = (const double*)dE;
+      P.Hshift = (const double*)dHs;
+    }
+    launch_prep_ens(NP, P, c->stream);
+    check(c, hipGetLastError(), "prep");
+    if (c->timing) c->timer.mark(PH_SELECT);
+    SelDev sel{};
+    int64_t total = 0;
+    check(c, run_selection(c->ws, g.zd, g.od, 1, c->stream, c->d_err, sel, &total), "selection");
+    T* dEa = dev ? Ea : (T*)c->ws.get(Workspace::OUT_S, sizeof(T) * std::max<int64_t>(n, 1) * N);
+    const int64_t ld_ea = dev ? ldEa : n;
+    int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+    if (!dEa || !sweeps) check(c, hipErrorOutOfMemory, "workspace");
+    ChunkArgs A{};
+    A.N = N;
+    A.mode = 1;
+    A.m = m;
+    A.zone_ptr = g.zd.zone_ptr;
+    A.sel_ptr = sel.ptr;
+    A.sel_idx = sel.idx;
+    A.sel_w = sel.w;
+    A.rho = rho;
+    A.innov = innov;
+    A.HS = HS;
+    A.ampl = nullptr;
+    A.err = c->d_err;
+    A.tol = c->jacobi_tol > 0 ? c->jacobi_tol : 4.0 * NP * 2.220446049250313e-16;
+    A.max_sweeps = c->max_sweeps;
+    A.X0 = xbar;
+    A.ldS = ld_e;
+    A.xa = nullptr;
+    A.ldO = ld_ea;
+    A.prec = mixed ? 1 : 0;
+    if (mixed) {
+      A.HS32 = HS32;
+      A.S32 = (const float*)dE;
+      A.Out32 = (float*)dEa;
+    } else {
+      A.S = (const double*)dE;
+      A.Out = (double*)dEa;
+    }
+    A.scaling = scaling;
+    A.infl = inflation;
+    A.maxcorr = max_correction;
+    run_zones(c, NP, A, nz, sweeps);
+    if (c->timing) c->timer.mark(PH_END);
+    check(c, hipEventRecord(e1, c->stream), "event");
+    finish_call(c, nz, sel.ptr, sweeps);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    collect_stats(c, nz, sel.ptr, ms);
+    if (!dev && n > 0)
+      check(c,
+            hipMemcpy2D(Ea, sizeof(T) * ldEa, dEa, sizeof(T) * n, sizeof(T) * n, N,
+                        hipMemcpyDeviceToHost),
+            "D2H Ea");
+    return OAKG_OK;
+    });
+}
+
 }  // namespace
 
 extern "C" {
@@ -633,118 +787,16 @@ int oakg_ensemble_analysis(oakg_ctx* c, const oakg_zones* zones, const oakg_obs*
                            const oakg_csr* H, const double* Hshift, int32_t N, const double* Ef,
                            int64_t ldEf, double* Ea, int64_t ldEa, double inflation,
                            double max_correction, uint32_t flags) {
-  return guarded(c, [&] {
-    const bool dev = flags & OAKG_DEVICE_PTRS;
-    if (!zones || !obs || !H || !Ef || !Ea) arg_fail(c, "zones, obs, H, Ef and Ea are required");
-    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
-    const int NP = padded_n(N);
-    if (!NP) {
-      c->err = "N above 128 is not supported by this build";
-      return OAKG_EUNSUP;
-    }
-    validate_zones_host(c, zones, 0, true);
-    validate_obs(c, obs, true, zones->loctype);
-    const int64_t nz = zones->nzones, m = obs->m;
-    if (H->m != m) arg_fail(c, "H->m must equal obs->m");
-    if (m > 0 && (!H->row_ptr || (H->nnz > 0 && (!H->col || !H->val)) || !Hshift))
-      arg_fail(c, "H arrays and Hshift are required");
-    int64_t n = 0;
-    if (!dev) {
-      validate_zone_ptr_host(c, nz, zones->zone_ptr);
-      n = zones->zone_ptr[nz];
-      if (H->row_ptr[0] != 0 || H->row_ptr[m] != H->nnz) arg_fail(c, "H->row_ptr inconsistent");
-      for (int64_t e = 0; e < H->nnz; ++e)
-        if (H->col[e] < 0 || H->col[e] >= n) arg_fail(c, "H column out of range");
-    } else if (nz > 0) {
-      check(c, hipMemcpy(&n, zones->zone_ptr + nz, sizeof(int64_t), hipMemcpyDeviceToHost), "D2H n");
-    }
-    if (H->n != n) arg_fail(c, "H->n must equal the state size");
-    if (ldEf < n || ldEa < n) arg_fail(c, "leading dimension too small");
-    if (!(inflation > 0) || !(max_correction > 0)) arg_fail(c, "inflation and max_correction must be > 0");
-    begin_call(c);
-    if (c->timing) c->timer.mark(PH_PREP);
-    hipEvent_t e0, e1;
-    check(c, hipEventCreate(&e0), "event");
-    check(c, hipEventCreate(&e1), "event");
-    check(c, hipEventRecord(e0, c->stream), "event");
-    Geo g = stage_geo(c, zones, obs, dev, true);
-    int64_t ld_e = 0;
-    const double* dE = stage2d(c, Workspace::IN_S, Ef, n, N, ldEf, dev, &ld_e);
-    const int64_t* drp = stage(c, Workspace::H_RP, H->row_ptr, m + 1, dev);
-    const int32_t* dcol = stage(c, Workspace::H_COL, H->col, H->nnz, dev);
-    const double* dval = stage(c, Workspace::H_VAL, H->val, H->nnz, dev);
-    const double* dHs = stage(c, Workspace::H_SHIFT, Hshift, m, dev);
-    double* xbar = (double*)c->ws.get(Workspace::XBAR, sizeof(double) * std::max<int64_t>(n, 1));
-    double* HEc = (double*)c->ws.get(Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
-    double* HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
-    double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
-    double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
-    if (!xbar || !HEc || !HS || !innov || !rho) check(c, hipErrorOutOfMemory, "workspace");
-    const double scaling = sqrt((double)N - 1.0);
-    PrepEnsArgs P{n, m, N, dE, ld_e, drp, dcol, dval, dHs, g.od.yo, g.od.rmse2, g.od.excluded,
-                  scaling, xbar, HEc, HS, innov, rho, c->d_err};
-    launch_prep_ens(NP, P, c->stream);
-    check(c, hipGetLastError(), "prep");
-    if (c->timing) c->timer.mark(PH_SELECT);
-    SelDev sel{};
-    int64_t total = 0;
-    check(c, run_selection(c->ws, g.zd, g.od, 1, c->stream, c->d_err, sel, &total), "selection");
-    double* dEa = dev ? Ea : (double*)c->ws.get(Workspace::OUT_S, sizeof(double) * std::max<int64_t>(n, 1) * N);
-    const int64_t ld_ea = dev ? ldEa : n;
-    int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
-    if (!dEa || !sweeps) check(c, hipErrorOutOfMemory, "workspace");
-    ChunkArgs A{};
-    A.N = N;
-    A.mode = 1;
-    A.m = m;
-    A.zone_ptr = g.zd.zone_ptr;
-    A.sel_ptr = sel.ptr;
-    A.sel_idx = sel.idx;
-    A.sel_w = sel.w;
-    A.rho = rho;
-    A.innov = innov;
-    A.HS = HS;
-    A.ampl = nullptr;
-    A.err = c->d_err;
-    A.tol = c->jacobi_tol > 0 ? c->jacobi_tol : 4.0 * NP * 2.220446049250313e-16;
-    A.max_sweeps = c->max_sweeps;
-    A.X0 = xbar;
-    A.S = dE;
-    A.ldS = ld_e;
-    A.xa = nullptr;
-    A.Out = dEa;
-    A.ldO = ld_ea;
-    A.scaling = scaling;
-    A.infl = inflation;
-    A.maxcorr = max_correction;
-    run_zones(c, NP, A, nz, sweeps);
-    if (c->timing) c->timer.mark(PH_END);
-    check(c, hipEventRecord(e1, c->stream), "event");
-    finish_call(c, nz, sel.ptr, sweeps);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    collect_stats(c, nz, sel.ptr, ms);
-    if (!dev && n > 0)
-      check(c,
-            hipMemcpy2D(Ea, sizeof(double) * ldEa, dEa, sizeof(double) * n, sizeof(double) * n, N,
-                        hipMemcpyDeviceToHost),
-            "D2H Ea");
-    return OAKG_OK;
-  });
+  return ensemble_impl<double>(c, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, inflation,
+                               max_correction, flags);
 }
 
 int oakg_ensemble_analysis_f32(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs,
                                const oakg_csr* H, const float* Hshift, int32_t N, const float* Ef,
                                int64_t ldEf, float* Ea, int64_t ldEa, double inflation,
                                double max_correction, uint32_t flags) {
-  return guarded(c, [&] {
-    (void)zones; (void)obs; (void)H; (void)Hshift; (void)N; (void)Ef; (void)ldEf; (void)Ea;
-    (void)ldEa; (void)inflation; (void)max_correction; (void)flags;
-    c->err = "mixed-precision path not built yet";
-    return OAKG_EUNSUP;
-  });
+  return ensemble_impl<float>(c, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, inflation,
+                              max_correction, flags);
 }
 
 int oakg_coo_to_csr(int64_t m, int64_t nnz, const int32_t* row, const int32_t* col,

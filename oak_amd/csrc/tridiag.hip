@@ -28,155 +28,245 @@ __device__ __forceinline__ double wave_sum_d(double s) {
 }
 
 // ---------------------------------------------------------------------------
+// Householder tridiagonalisation (lower, as dsytd2) with the matrix in
+// registers: thread t of 256 holds the T x T tile (rows T*(t/16).., columns
+// T*(t%16)..), T = NP/16.  Each step extracts column i through LDS, builds
+// v and tau in wave 0, forms p = tau A v from per-tile partial sums, and
+// applies A -= v w^T + w v^T (w = p - tau/2 (p.v) v) to the tiles that meet
+// the trailing block.  The reflectors go to the rows of qmat; Q is then
+// accumulated backwards in the same tile layout and overwrites them.
+// T and Q are written in reversed order (d'_k = d_{NP-1-k}, columns of Q
+// reversed): the reduction from the top leaves the large diagonal entries at
+// the top of T, and QL, which deflates at the top of each block, converges
+// slowly on such grading (tens of sweeps per eigenvalue for strongly
+// observed zones; LAPACK's dsteqr switches to QR there).  Reversed, the
+// small entries come first and QL needs a few sweeps per eigenvalue.
+// ---------------------------------------------------------------------------
+template <int NP>
+struct TriLds {
+  double col[2][NP];       // column i / Householder vector v (double-buffered)
+  double pw[2][NP];        // p (then y during the Q accumulation)
+  double part[16][NP + 1]; // per-tile partial sums
+  double red[2][4];        // wave partial dot products (double-buffered)
+  double tau[NP];
+};
+
 template <int NP>
 __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __restrict__ sel_ptr,
                                                  const double* __restrict__ mat,
                                                  double* __restrict__ qmat,
                                                  double* __restrict__ dvec, double* __restrict__ evec,
                                                  int nzc) {
-  constexpr int LD = NP + 1;
-  __shared__ double A[NP * LD];
-  __shared__ double pv[NP];
-  __shared__ double tau[NP];
-  __shared__ double sc[4];
+  constexpr int T = NP / 16;
+  constexpr int NW = NP >= 64 ? NP / 64 : 1;  // waves that hold one entry of a length-NP vector
+  __shared__ TriLds<NP> S;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int t = threadIdx.x, tr = t >> 4, tc = t & 15, wave = t >> 6, lane = t & 63;
+  const int r0 = tr * T, c0 = tc * T;
   const double* M = mat + (size_t)zl * NP * NP;
-  for (int e = tid; e < NP * NP; e += 256) A[(e / NP) * LD + e % NP] = M[e];
-  __syncthreads();
-  // ---- reduction to tridiagonal form, column by column ----
+  double* V = qmat + (size_t)zl * NP * NP;
+  double a[T][T];
+#pragma unroll
+  for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < T; ++jj) a[ii][jj] = M[(r0 + ii) * NP + c0 + jj];
+
   for (int i = 0; i < NP - 2; ++i) {
-    const int L = NP - 1 - i;  // rows/columns i+1 .. NP-1
-    if (wave == 0) {
-      double s = 0;
-      for (int r = i + 2 + lane; r < NP; r += 64) {
-        const double x = A[r * LD + i];
-        s = fma(x, x, s);
-      }
-      s = wave_sum_d(s);
-      if (lane == 0) {
-        const double alpha = A[(i + 1) * LD + i];
-        double t = 0.0, beta = alpha, scal = 0.0;
-        if (s > 0.0) {
-          const double nrm = sqrt(fma(alpha, alpha, s));
-          beta = alpha >= 0.0 ? -nrm : nrm;
-          t = (beta - alpha) / beta;
-          scal = 1.0 / (alpha - beta);
+    const int b = i & 1;
+    double* col = S.col[b];
+    double* pw = S.pw[b];
+    // (1) column i, all rows, from the 16 threads of tile column i / T
+    if (tc == i / T) {
+      const int j0 = i % T;
+#pragma unroll
+      for (int jj = 0; jj < T; ++jj)
+        if (jj == j0) {
+#pragma unroll
+          for (int ii = 0; ii < T; ++ii) col[r0 + ii] = a[ii][jj];
         }
-        sc[0] = t;
-        sc[1] = beta;
-        sc[2] = scal;
-      }
     }
     __syncthreads();
-    const double t = sc[0], scal = sc[2];
-    if (t != 0.0)
-      for (int r = i + 2 + tid; r < NP; r += 256) A[r * LD + i] *= scal;
-    if (tid == 0) {
-      dvec[(size_t)i * nzc + zl] = A[i * LD + i];
-      evec[(size_t)i * nzc + zl] = sc[1];
-      tau[i] = t;
-      A[(i + 1) * LD + i] = 1.0;  // v(i+1) = 1
-    }
-    __syncthreads();
-    if (t == 0.0) continue;
-    // p = t A_sub v, two threads per row
-    {
-      const int rr = tid >> 1, h = tid & 1;
-      double sum = 0;
-      if (rr < L) {
-        const int r = i + 1 + rr;
-        const int half = (L + 1) >> 1;
-        const int c0 = i + 1 + h * half;
-        const int c1 = min(i + 1 + (h + 1) * half, NP);
-        for (int c = c0; c < c1; ++c) sum = fma(A[r * LD + c], A[c * LD + i], sum);
-      }
-      sum += shfl_xor_d(sum, 1);
-      if (rr < L && h == 0) pv[i + 1 + rr] = t * sum;
-    }
-    __syncthreads();
+    // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i]
     if (wave == 0) {
       double s = 0;
-      for (int r = i + 1 + lane; r < NP; r += 64) s = fma(pv[r], A[r * LD + i], s);
+      for (int r = i + 2 + lane; r < NP; r += 64) s = fma(col[r], col[r], s);
       s = wave_sum_d(s);
-      if (lane == 0) sc[3] = -0.5 * t * s;
-    }
-    __syncthreads();
-    const double K = sc[3];
-    // A_sub -= v w^T + w v^T, w = p + K v
-    for (int e = tid; e < L * L; e += 256) {
-      const int r = i + 1 + e / L, c = i + 1 + e % L;
-      const double vr = A[r * LD + i], vc = A[c * LD + i];
-      const double wr = fma(K, vr, pv[r]), wc = fma(K, vc, pv[c]);
-      A[r * LD + c] -= fma(vr, wc, wr * vc);
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    dvec[(size_t)(NP - 2) * nzc + zl] = A[(NP - 2) * LD + NP - 2];
-    evec[(size_t)(NP - 2) * nzc + zl] = A[(NP - 1) * LD + NP - 2];
-    dvec[(size_t)(NP - 1) * nzc + zl] = A[(NP - 1) * LD + NP - 1];
-    evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
-  }
-  __syncthreads();
-  // ---- Q = H_0 H_1 ... H_{NP-3}, backward accumulation in place ----
-  if (tid < 4) A[(NP - 2 + tid / 2) * LD + NP - 2 + tid % 2] = (tid / 2 == tid % 2) ? 1.0 : 0.0;
-  __syncthreads();
-  for (int i = NP - 3; i >= 0; --i) {
-    const int a0 = i + 1;
-    // row/column a0 of the active block become the identity's
-    for (int r = a0 + 1 + tid; r < NP; r += 256) {
-      A[r * LD + a0] = 0.0;
-      A[a0 * LD + r] = 0.0;
-    }
-    if (tid == 0) A[a0 * LD + a0] = 1.0;
-    __syncthreads();
-    const double ti = tau[i];
-    if (ti == 0.0) continue;
-    const int L = NP - a0;
-    // y_c = sum_r v_r Q[r][c]  (two threads per column)
-    {
-      const int cc = tid >> 1, h = tid & 1;
-      double sum = 0;
-      if (cc < L) {
-        const int c = a0 + cc;
-        const int half = (L + 1) >> 1;
-        const int r0 = a0 + h * half;
-        const int r1 = min(a0 + (h + 1) * half, NP);
-        for (int r = r0; r < r1; ++r) sum = fma(A[r * LD + i], A[r * LD + c], sum);
+      const double alpha = col[i + 1];
+      const double dii = col[i];
+      double tau = 0.0, beta = alpha, scal = 0.0;
+      if (s > 0.0) {
+        const double nrm = sqrt(fma(alpha, alpha, s));
+        beta = alpha >= 0.0 ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
       }
-      sum += shfl_xor_d(sum, 1);
-      if (cc < L && h == 0) pv[a0 + cc] = ti * sum;
+      for (int r = lane; r < NP; r += 64) {
+        const double v = r <= i ? 0.0 : (r == i + 1 ? 1.0 : col[r] * scal);
+        col[r] = v;
+        V[(size_t)i * NP + r] = v;
+      }
+      if (lane == 0) {
+        dvec[(size_t)(NP - 1 - i) * nzc + zl] = dii;
+        evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
+        S.tau[i] = tau;
+      }
     }
     __syncthreads();
-    for (int e = tid; e < L * L; e += 256) {
-      const int r = a0 + e / L, c = a0 + e % L;
-      A[r * LD + c] -= A[r * LD + i] * pv[c];
+    const double tau = S.tau[i];
+    // (3) per-tile partial sums of A v (v vanishes on rows/columns <= i)
+    const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
+    {
+      double vc[T];
+#pragma unroll
+      for (int jj = 0; jj < T; ++jj) vc[jj] = act ? col[c0 + jj] : 0.0;
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii) {
+        double s = 0;
+        if (act) {
+#pragma unroll
+          for (int jj = 0; jj < T; ++jj) s = fma(a[ii][jj], vc[jj], s);
+        }
+        S.part[tc][r0 + ii] = s;
+      }
     }
     __syncthreads();
+    // (4) p = tau A v and p.v
+    {
+      double pr = 0, vr = 0;
+      if (t < NP) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s += S.part[q][t];
+        pr = t > i ? tau * s : 0.0;
+        vr = col[t];
+        pw[t] = pr;
+      }
+      if (wave < NW) {
+        const double dsum = wave_sum_d(pr * vr);
+        if (lane == 0) S.red[b][wave] = dsum;
+      }
+    }
+    __syncthreads();
+    // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
+    if (act) {
+      double pv = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) pv += S.red[b][q];
+      const double K = -0.5 * tau * pv;
+      double vR[T], wR[T], vC[T], wC[T];
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii) {
+        vR[ii] = col[r0 + ii];
+        wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < T; ++jj) {
+        vC[jj] = col[c0 + jj];
+        wC[jj] = fma(K, vC[jj], pw[c0 + jj]);
+      }
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < T; ++jj) a[ii][jj] -= fma(vR[ii], wC[jj], wR[ii] * vC[jj]);
+    }
   }
-  for (int r = 1 + tid; r < NP; r += 256) {
-    A[r * LD] = 0.0;
-    A[r] = 0.0;
+  // trailing 2 x 2 block
+#pragma unroll
+  for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < T; ++jj) {
+      const int r = r0 + ii, c = c0 + jj;
+      if (c == NP - 2 && r == NP - 2) dvec[(size_t)1 * nzc + zl] = a[ii][jj];
+      if (c == NP - 2 && r == NP - 1) evec[(size_t)0 * nzc + zl] = a[ii][jj];
+      if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a[ii][jj];
+    }
+  if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
+
+  // ---- Q = H_0 H_1 ... H_{NP-3}, accumulated backwards: Q <- H_k Q ----
+  double q[T][T];
+#pragma unroll
+  for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < T; ++jj) q[ii][jj] = (r0 + ii == c0 + jj) ? 1.0 : 0.0;
+  __syncthreads();  // reflectors written by wave 0 are visible to the workgroup
+  double vnext = (t < NP) ? V[(size_t)(NP - 3) * NP + t] : 0.0;
+  for (int k = NP - 3; k >= 0; --k) {
+    const int b = k & 1;
+    double* vb = S.col[b];
+    double* yb = S.pw[b];
+    if (t < NP) vb[t] = vnext;
+    if (k > 0 && t < NP) vnext = V[(size_t)(k - 1) * NP + t];  // prefetch
+    __syncthreads();
+    const bool act = (r0 + T - 1 > k) && (c0 + T - 1 > k);
+    double vR[T];
+#pragma unroll
+    for (int ii = 0; ii < T; ++ii) vR[ii] = act ? vb[r0 + ii] : 0.0;
+#pragma unroll
+    for (int jj = 0; jj < T; ++jj) {
+      double s = 0;
+      if (act) {
+#pragma unroll
+        for (int ii = 0; ii < T; ++ii) s = fma(vR[ii], q[ii][jj], s);
+      }
+      S.part[tr][c0 + jj] = s;
+    }
+    __syncthreads();
+    if (t < NP) {
+      double s = 0;
+#pragma unroll
+      for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
+      yb[t] = S.tau[k] * s;
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < T; ++jj) q[ii][jj] = fma(-vR[ii], yb[c0 + jj], q[ii][jj]);
+    }
   }
-  if (tid == 0) A[0] = 1.0;
   __syncthreads();
-  double* Q = qmat + (size_t)zl * NP * NP;
-  for (int e = tid; e < NP * NP; e += 256) Q[e] = A[(e / NP) * LD + e % NP];
+#pragma unroll
+  for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < T; ++jj) V[(size_t)(r0 + ii) * NP + (NP - 1 - (c0 + jj))] = q[ii][jj];
 }
 
 // ---------------------------------------------------------------------------
-// Rotation log: per zone, records of two doubles.  A chain (one QL sweep on
-// rows l..m) starts with a header {l + 65536 m, count} followed by `count`
-// records {c, s} for indices i = m-1, m-2, ...
+// Rotation log.  The zones of one 64-zone group share cap * 64 records; record
+// k of the zone in lane l sits at (k / 4) * 256 + 4 l + (k % 4), so that one
+// store instruction of the QL wave stays within a few KB.  A chain (one QL
+// sweep on rows l..m) starts with a header {l + 65536 m, count} followed by
+// `count` records {c, s} for the rotation indices i = m-1, m-2, ...
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t rlog_off(int zl, int64_t cap) {
+  return (size_t)(zl >> 6) * (size_t)cap * 64 + (size_t)(zl & 63) * 4;
+}
+__device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) * 256 + (size_t)(k & 3); }
+
+// 1/sqrt(h) to full double precision: hardware estimate + two Newton steps
+__device__ __forceinline__ double rsqrt_acc(double h) {
+  double y = __builtin_amdgcn_rsq(h);
+  double hy = h * y;
+  y = fma(y * 0.5, fma(-hy, y, 1.0), y);
+  hy = h * y;
+  y = fma(y * 0.5, fma(-hy, y, 1.0), y);
+  return y;
+}
+
+// Implicit QL with Wilkinson shifts on (d, e), one lane per zone (EISPACK
+// tql2 ordering: eigenvalue l converges at the top of the block l..m).  The
+// end m of the unreduced block is carried from one sweep to the next: the
+// negligibility test of e_j is made as soon as e_j, d_j and d_{j+1} are final
+// in the sweep, so a full rescan is only needed after a 1 x 1 deflation.
 template <int NP>
 __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* __restrict__ sel_ptr,
                                             double* __restrict__ dvec, double* __restrict__ evec,
                                             double2* __restrict__ rlog, int64_t cap,
-                                            int32_t* __restrict__ status) {
+                                            int32_t* __restrict__ status,
+                                            unsigned* __restrict__ err) {
   __shared__ double d[NP][64];
   __shared__ double e[NP][64];
   const int lane = threadIdx.x;
@@ -190,104 +280,192 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
   }
   int st = 0;
   int64_t pos = 0;
-  double2* lg = rlog + (size_t)(valid ? zl : 0) * cap;
+  double2* lg = rlog + (valid ? rlog_off(zl, cap) : 0);
   const double eps = 2.220446049250313e-16;
   if (valid) {
-    for (int l = 0; l < NP && st == 0; ++l) {
-      int iter = 0;
-      for (;;) {
-        int m = l;
-        for (; m < NP - 1; ++m) {
-          const double dd = fabs(d[m][lane]) + fabs(d[m + 1][lane]);
-          if (fabs(e[m][lane]) <= eps * dd) break;
+    auto negl = [&](int j) {
+      return fabs(e[j][lane]) <= eps * (fabs(d[j][lane]) + fabs(d[j + 1][lane]));
+    };
+    auto scan = [&](int from) {
+      int m = from;
+      for (; m < NP - 1; ++m)
+        if (negl(m)) break;
+      return m;
+    };
+    int l = 0, iter = 0;
+    int m = scan(0);
+    for (;;) {
+      if (m == l) {
+        ++l;
+        if (l >= NP - 1) break;
+        iter = 0;
+        m = scan(l);
+        continue;
+      }
+      if (++iter > 60) { st = 2; break; }
+      if (pos + 1 + (m - l) > cap) { st = 1; break; }
+      const double el = e[l][lane], dl = d[l][lane];
+      double g = (d[l + 1][lane] - dl) / (2.0 * el);
+      const double rg = sqrt(fma(g, g, 1.0));
+      g = d[m][lane] - dl + el / (g + (g >= 0 ? rg : -rg));
+      double s = 1.0, c = 1.0, p = 0.0;
+      const int64_t hdr = pos++;
+      int i = m - 1;
+      double ei = e[i][lane], di = d[i][lane], dip1 = d[i + 1][lane];
+      double dnext2 = 0.0;  // new d_{i+2}
+      int jmin = m;
+      bool under = false;
+      for (; i >= l; --i) {
+        double ein = 0.0, din = 0.0;
+        if (i > l) {  // operands of the next step, loaded ahead
+          ein = e[i - 1][lane];
+          din = d[i - 1][lane];
         }
-        if (m == l) break;
-        if (++iter > 60) { st = 2; break; }
-        if (pos + 1 + (m - l) > cap) { st = 1; break; }
-        double g = (d[l + 1][lane] - d[l][lane]) / (2.0 * e[l][lane]);
-        double r = sqrt(fma(g, g, 1.0));
-        g = d[m][lane] - d[l][lane] + e[l][lane] / (g + (g >= 0 ? r : -r));
-        double s = 1.0, c = 1.0, p = 0.0;
-        const int64_t hdr = pos++;
-        int cnt = 0;
-        bool under = false;
-        for (int i = m - 1; i >= l; --i) {
-          const double f = s * e[i][lane], b = c * e[i][lane];
-          r = sqrt(fma(f, f, g * g));
-          e[i + 1][lane] = r;
-          if (r == 0.0) {
-            d[i + 1][lane] -= p;
-            e[m][lane] = 0.0;
-            under = true;
-            break;
-          }
-          const double ri = 1.0 / r;
-          s = f * ri;
-          c = g * ri;
-          g = d[i + 1][lane] - p;
-          r = fma(d[i][lane] - g, s, 2.0 * c * b);
-          p = s * r;
-          d[i + 1][lane] = g + p;
-          g = fma(c, r, -b);
-          lg[pos++] = make_double2(c, s);
-          ++cnt;
+        const double f = s * ei, bb = c * ei;
+        const double h = fma(f, f, g * g);
+        if (h == 0.0) {  // exact zero: the block splits at i + 1
+          d[i + 1][lane] = dip1 - p;
+          e[i + 1][lane] = 0.0;
+          e[m][lane] = 0.0;
+          under = true;
+          break;
         }
-        lg[hdr] = make_double2((double)l + 65536.0 * (double)m, (double)cnt);
-        if (under) continue;
-        d[l][lane] -= p;
-        e[l][lane] = g;
-        e[m][lane] = 0.0;
+        const double ri = rsqrt_acc(h);
+        const double r = h * ri;
+        e[i + 1][lane] = r;
+        s = f * ri;
+        c = g * ri;
+        g = dip1 - p;
+        const double rr = fma(di - g, s, 2.0 * c * bb);
+        p = s * rr;
+        const double dn = g + p;
+        d[i + 1][lane] = dn;
+        g = fma(c, rr, -bb);
+        lg[rlog_rec(pos)] = make_double2(c, s);
+        ++pos;
+        if (i + 1 < m && fabs(r) <= eps * (fabs(dn) + fabs(dnext2))) jmin = i + 1;
+        dnext2 = dn;
+        ei = ein;
+        dip1 = di;
+        di = din;
+      }
+      lg[rlog_rec(hdr)] = make_double2((double)l + 65536.0 * (double)m, (double)(pos - hdr - 1));
+      if (under) {
+        m = i + 1;
+        continue;
+      }
+      const double dl2 = dl - p;
+      d[l][lane] = dl2;
+      e[l][lane] = g;
+      e[m][lane] = 0.0;
+      if (fabs(g) <= eps * (fabs(dl2) + fabs(dnext2))) {
+        ++l;
+        if (l >= NP - 1) break;
+        iter = 0;
+        m = jmin > l ? jmin : scan(l);
+      } else {
+        m = jmin;
       }
     }
     for (int k = 0; k < NP; ++k) dvec[(size_t)k * nzc + zl] = d[k][lane];
   }
   if (zl < nzc) status[zl] = valid ? (st | (int)(pos << 2)) : 0;
+  if (st) atomicAdd(err + 1, 1u);
 }
 
 // ---------------------------------------------------------------------------
+// U = Q Z: the logged rotations applied to the rows of Q held in LDS, one
+// thread per row.  The header and records of the next chain are loaded
+// while the current one is applied.
+// ---------------------------------------------------------------------------
 template <int NP>
-__global__ __launch_bounds__(256) void k_tql_apply(int64_t z0, const int64_t* __restrict__ sel_ptr,
-                                                   double* __restrict__ qmat,
-                                                   const double2* __restrict__ rlog, int64_t cap,
-                                                   const int32_t* __restrict__ status) {
+__global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
+                                                                 const int64_t* __restrict__ sel_ptr,
+                                                                 double* __restrict__ qmat,
+                                                                 const double2* __restrict__ rlog,
+                                                                 int64_t cap,
+                                                                 const int32_t* __restrict__ status) {
   constexpr int LD = NP + 1;
-  __shared__ double Z[NP * LD];
-  __shared__ double2 cs[NP];
+  constexpr int BS = NP < 64 ? 64 : NP;
+  constexpr int PAD = 8;  // room for the look-ahead reads below column 0 of row 0
+  __shared__ double Zs[PAD + NP * LD];
+  __shared__ double2 buf[2][NP + 12];
+  double* Z = Zs + PAD;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
-  if (status[zl] & 3) return;  // failed zone: handled by the caller
+  const int stz = status[zl];
+  if (stz & 3) return;  // failed zone: solved again by the fallback
   const int tid = threadIdx.x;
   double* Q = qmat + (size_t)zl * NP * NP;
-  for (int e = tid; e < NP * NP; e += 256) Z[(e / NP) * LD + e % NP] = Q[e];
-  const int64_t len = status[zl] >> 2;
-  const double2* lg = rlog + (size_t)zl * cap;
+  for (int e = tid; e < NP * NP; e += BS) Z[(e / NP) * LD + e % NP] = Q[e];
+  const int64_t len = stz >> 2;
+  const double2* lg = rlog + rlog_off(zl, cap);
+  if (tid < NP && tid < len) buf[0][tid] = lg[rlog_rec(tid)];
   __syncthreads();
   int64_t pos = 0;
+  int b = 0;
   while (pos < len) {
-    const double2 h = lg[pos];
-    const int lm = (int)h.x;
-    const int l = lm & 65535, m = lm >> 16;
-    const int cnt = (int)h.y;
-    for (int k = tid; k < cnt; k += 256) cs[k] = lg[pos + 1 + k];
-    __syncthreads();
-    if (tid < NP && cnt > 0) {
-      double* zr = Z + tid * LD;
-      double carry = zr[m];
-      for (int k = 0; k < cnt; ++k) {
-        const int i = m - 1 - k;
-        const double c = cs[k].x, s = cs[k].y;
-        const double zi = zr[i];
-        zr[i + 1] = fma(s, zi, c * carry);
-        carry = fma(c, zi, -s * carry);
+    const double2 h = buf[b][0];
+    const int m = __builtin_amdgcn_readfirstlane(((int)h.x) >> 16);
+    const int cnt = __builtin_amdgcn_readfirstlane((int)h.y);
+    const int64_t nxt = pos + 1 + cnt;
+    const bool pf = tid < NP && nxt + tid < len;
+    double2 pv = make_double2(0.0, 0.0);
+    if (pf) pv = lg[rlog_rec(nxt + tid)];
+    if (tid < NP) {
+      // column i of this row is p[0]; rotations i = m-1, m-2, ... with
+      // (c, s) = CS(0), CS(1), ...  Groups of 4 rotations in two register
+      // sets; the operands of the next group are read while the current one
+      // is applied.  The look-ahead reads reach at most 4 columns below the
+      // chain (into the row before, or PAD) and 11 records past it (buf).
+      typedef __attribute__((address_space(3))) const double lds_d;
+      double* p = Z + tid * LD + (m - 1);
+      lds_d* cd = (lds_d*)(buf[b] + 1);
+      asm volatile("" : "+v"(cd));  // one VGPR base for the broadcast reads
+      double carry = p[1];
+      int k = 0;
+#define CS(J) make_double2(cd[2 * (J)], cd[2 * (J) + 1])
+#define OAK_ROT(Q, ZV, DST)                  \
+  DST = fma(Q.y, ZV, Q.x * carry);           \
+  carry = fma(Q.x, ZV, -Q.y * carry);
+      if (cnt >= 8) {
+        double a0 = p[0], a1 = p[-1], a2 = p[-2], a3 = p[-3];
+        double2 qa0 = CS(0), qa1 = CS(1), qa2 = CS(2), qa3 = CS(3);
+        for (; k + 8 <= cnt; k += 8) {
+          const double b0 = p[-4], b1 = p[-5], b2 = p[-6], b3 = p[-7];
+          const double2 qb0 = CS(4), qb1 = CS(5), qb2 = CS(6), qb3 = CS(7);
+          OAK_ROT(qa0, a0, p[1])
+          OAK_ROT(qa1, a1, p[0])
+          OAK_ROT(qa2, a2, p[-1])
+          OAK_ROT(qa3, a3, p[-2])
+          a0 = p[-8]; a1 = p[-9]; a2 = p[-10]; a3 = p[-11];
+          qa0 = CS(8); qa1 = CS(9); qa2 = CS(10); qa3 = CS(11);
+          OAK_ROT(qb0, b0, p[-3])
+          OAK_ROT(qb1, b1, p[-4])
+          OAK_ROT(qb2, b2, p[-5])
+          OAK_ROT(qb3, b3, p[-6])
+          p -= 8;
+          cd += 16;
+        }
       }
-      zr[m - cnt] = carry;
+      for (; k < cnt; ++k) {
+        const double zv = p[0];
+        const double2 q = CS(0);
+        OAK_ROT(q, zv, p[1])
+        p -= 1;
+        cd += 2;
+      }
+#undef OAK_ROT
+#undef CS
+      p[1] = carry;  // column m - cnt
     }
-    (void)l;
-    pos += 1 + cnt;
+    if (pf) buf[b ^ 1][tid] = pv;
     __syncthreads();
+    pos = nxt;
+    b ^= 1;
   }
-  for (int e = tid; e < NP * NP; e += 256) Q[e] = Z[(e / NP) * LD + e % NP];
+  for (int e = tid; e < NP * NP; e += BS) Q[e] = Z[(e / NP) * LD + e % NP];
 }
 
 // ---------------------------------------------------------------------------
@@ -390,8 +568,8 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   hipLaunchKernelGGL(k_tridiag<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.mat, A.ymat,
                      A.dvec, A.evec, A.nzc);
   hipLaunchKernelGGL(k_tql<NP>, dim3((A.nzc + 63) / 64), dim3(64), 0, st, A.nzc, A.z0, A.sel_ptr,
-                     A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status);
-  hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.ymat,
+                     A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
+  hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0, A.sel_ptr, A.ymat,
                      (const double2*)A.rlog, A.rlog_cap, A.tql_status);
   hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
                      A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);

@@ -63,7 +63,22 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
-  const int t = threadIdx.x, tr = t >> 4, tc = t & 15, wave = t >> 6, lane = t & 63;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  // Tiles in decreasing order of min(tile row, tile column): the tiles that
+  // meet the trailing block [i+1, NP)^2 are always threads 0 .. (16-k)^2 - 1
+  // (k = the smallest active tile index), so whole waves fall idle as the
+  // block shrinks instead of idling a few lanes in every wave.  Thread t
+  // lies on ring s = floor(sqrt t) (tiles with min index 15 - s), position
+  // j = t - s^2: j <= s on the ring's row, j > s on its column.
+  int tr, tc;
+  {
+    int s = (int)sqrtf((float)t);
+    while (s * s > t) --s;
+    while ((s + 1) * (s + 1) <= t) ++s;
+    const int j = t - s * s, mn = 15 - s;
+    if (j <= s) { tr = mn; tc = mn + j; }
+    else { tr = mn + (j - s); tc = mn; }
+  }
   const int r0 = tr * T, c0 = tc * T;
   const double* M = mat + (size_t)zl * NP * NP;
   double* V = qmat + (size_t)zl * NP * NP;

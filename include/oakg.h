@@ -181,6 +181,26 @@ int oakg_ensemble_analysis_f32(oakg_ctx* ctx, const oakg_zones* zones, const oak
 
 /* Host helpers ------------------------------------------------------------ */
 /* Stable COO -> CSR by row (entries of a row keep their COO order). */
+/* Innovation negative log-likelihood of the local analysis and its gradient
+   (a hand-written adjoint; no counterpart in OAK, which has no
+   differentiation: it serves the tuning of the analysis hyperparameters).
+   With the selection and taper of oakg_local_analysis, per zone z:
+     J_z = 1/2 [ d^T D^-1 d + log det D ],  D = a^2 S S^T + diag(sigma^2 / w^2)
+   over the zone's observations (d = yo - Hxf, S = HSf rows, a = inflation),
+   evaluated through the ensemble-space eigendecomposition.  Outputs:
+     nll             sum_z J_z (1 value)
+     grad_log_sigma  dJ/dlog sigma_l, sigma_l^2 = obs->rmse2[l]   [m]
+     grad_corr_len   dJ/dL_z for the zone lengths zones->corr_len [nzones]
+                     (the cutoffs zones->max_len are held fixed)
+     grad_inflation  dJ/da (1 value)
+     nll_zone        J_z [nzones] (may be NULL)
+   grad_log_sigma is accumulated with atomic adds (order-dependent to the
+   last bits); the other outputs are deterministic. */
+int oakg_local_nll(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs, int32_t N,
+                   const double* Hxf, const double* HSf, int64_t ldHSf, double inflation,
+                   double* nll, double* grad_log_sigma, double* grad_corr_len,
+                   double* grad_inflation, double* nll_zone, uint32_t flags);
+
 int oakg_coo_to_csr(int64_t m, int64_t nnz, const int32_t* row, const int32_t* col,
                     const double* val, int64_t* row_ptr, int32_t* col_out, double* val_out);
 /* Zone layout from a partition vector: returns nzones (or -1), fills

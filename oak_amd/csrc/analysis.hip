@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void k_prep_loc(int64_t m, int N, const double
                                                   double* __restrict__ HS,
                                                   double* __restrict__ innov,
                                                   double* __restrict__ rho,
-                                                  unsigned* __restrict__ err) {
+                                                  unsigned* __restrict__ err, double scale) {
   __shared__ double tile[TR][NP + 1];
   const int64_t l0 = (int64_t)blockIdx.x * TR;
   for (int e = threadIdx.x; e < TR * NP; e += 256) {
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_prep_loc(int64_t m, int N, const double
   for (int e = threadIdx.x; e < TR * NP; e += 256) {
     int li = e / NP, k = e % NP;
     int64_t l = l0 + li;
-    if (l < m) HS[l * NP + k] = tile[li][k];
+    if (l < m) HS[l * NP + k] = tile[li][k] * scale;
   }
 }
 
@@ -432,6 +432,7 @@ __global__ __launch_bounds__(256) void k_eig(int64_t z0, int nzc, int N,
     double* V = vec + (size_t)zl * 4 * NP;
     V[NP + r] = active ? ar : 0.0;
     V[2 * NP + r] = vr;
+    V[3 * NP + r] = muc;
     if (ampl && active && r < N) ampl[(size_t)z * N + r] = ar;
     if (active && ar != ar) atomicOr(err, OAKG_DERR_NAN);
     if (r == 0) sweeps_out[zl] = sweeps;
@@ -614,7 +615,8 @@ template <int NP>
 static void prep_loc_t(const PrepLocArgs& A, hipStream_t st) {
   constexpr int TR = NP > 64 ? 32 : 64;
   hipLaunchKernelGGL((k_prep_loc<NP, TR>), dim3(nblk(A.m, TR)), dim3(256), 0, st, A.m, A.N, A.HSf,
-                     A.ldHSf, A.Hxf, A.yo, A.rmse2, A.excl, A.HS, A.innov, A.rho, A.err);
+                     A.ldHSf, A.Hxf, A.yo, A.rmse2, A.excl, A.HS, A.innov, A.rho, A.err,
+                     A.hs_scale);
 }
 
 void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st) {
@@ -673,6 +675,12 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
     hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.nzc,
                        A.N, A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
                        A.max_sweeps, (const int32_t*)nullptr);
+  }
+  if (A.mode == 2) {  // innovation likelihood: adjoint instead of transform + update
+    if (tm) tm->mark(PH_UPDATE);
+    launch_nll_adjoint(NP, A, st);
+    if (tm) tm->mark(PH_END);
+    return;
   }
   if (tm) tm->mark(PH_TMAT);
   hipLaunchKernelGGL(k_tmat<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat,

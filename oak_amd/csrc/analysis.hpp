@@ -49,6 +49,7 @@ struct PrepLocArgs {
   const uint8_t* excl;
   double *HS, *innov, *rho;
   unsigned* err;
+  double hs_scale;  // HS = hs_scale * HSf (1 for locAnalysis, the inflation for the likelihood)
 };
 
 struct PrepEnsArgs {
@@ -103,6 +104,10 @@ struct ChunkArgs {
   const float* HS32;
   const float* S32;
   float* Out32;
+  // mode 2 (innovation likelihood and its adjoint): per-zone L, per-zone
+  // outputs J_z, dJ/dL_z, dJ/dalpha_z and per-observation dJ/dlog sigma
+  const double* corr;
+  double *nll_J, *nll_gL, *nll_gA, *nll_gth;
 };
 
 void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
@@ -112,5 +117,8 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st);
+void launch_nll_adjoint(int NP, const ChunkArgs& A, hipStream_t st);
+void launch_nll_totals(int64_t nz, const double* Jz, const double* gA, double* out,
+                       hipStream_t st);
 
 }  // namespace oakg

@@ -431,6 +431,7 @@ __global__ __launch_bounds__(256) void k_eig128(int64_t z0, int nzc, int N,
   if (hh == 0) {
     V[P128 + jc] = ar;
     V[2 * P128 + jc] = vn;
+    V[3 * P128 + jc] = muc;
     if (ampl && jc < N) ampl[(size_t)z * N + jc] = ar;
     if (ar != ar) atomicOr(err, OAKG_DERR_NAN);
   }
@@ -604,6 +605,12 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
     hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.nzc, A.N, A.sel_ptr,
                        A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps,
                        (const int32_t*)nullptr);
+  }
+  if (A.mode == 2) {  // innovation likelihood: adjoint instead of transform + update
+    if (tm) tm->mark(PH_UPDATE);
+    launch_nll_adjoint(128, A, st);
+    if (tm) tm->mark(PH_END);
+    return;
   }
   if (tm) tm->mark(PH_TMAT);
   hipLaunchKernelGGL(k_tmat128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat, A.mat,

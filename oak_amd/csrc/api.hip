@@ -418,7 +418,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
   if (!HS || !innov || !rho) check(c, hipErrorOutOfMemory, "workspace");
   PrepLocArgs P{m, N, dHSf, ld_hs, dHxf, g.od.yo, g.od.rmse2, g.od.excluded, HS, innov, rho,
-                c->d_err};
+                c->d_err, 1.0};
   launch_prep_loc(NP, P, c->stream);
   check(c, hipGetLastError(), "prep");
 
@@ -511,6 +511,103 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
       check(c, hipMemcpy(ampl, d_ampl, sizeof(double) * (size_t)N * nz, hipMemcpyDeviceToHost),
             "D2H ampl");
   }
+  return OAKG_OK;
+}
+
+// Innovation negative log-likelihood of the local analysis and its gradient
+// with respect to log sigma (per observation), L (per zone) and the
+// inflation of the anomalies (see nll.hip).  Geometry-mode selection.
+int local_nll_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, int32_t N,
+                   const double* Hxf, const double* HSf, int64_t ldHSf, double inflation,
+                   double* nll, double* grad_log_sigma, double* grad_corr_len,
+                   double* grad_inflation, double* nll_zone, uint32_t flags) {
+  const bool dev = flags & OAKG_DEVICE_PTRS;
+  if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+  const int NP = padded_n(N);
+  if (!NP) {
+    c->err = "N above 128 is not supported by this build";
+    return OAKG_EUNSUP;
+  }
+  validate_zones_host(c, zones, 0, true);
+  validate_obs(c, obs, true, zones->loctype);
+  const int64_t nz = zones->nzones, m = obs->m;
+  if (!nll || !grad_log_sigma || !grad_corr_len || !grad_inflation)
+    arg_fail(c, "nll, grad_log_sigma, grad_corr_len and grad_inflation are required");
+  if (m > 0 && (!Hxf || !HSf)) arg_fail(c, "Hxf and HSf are required");
+  if (m > 0 && ldHSf < m) arg_fail(c, "leading dimension too small");
+  if (!(inflation > 0) || !std::isfinite(inflation)) arg_fail(c, "inflation must be > 0");
+  if (!dev) validate_zone_ptr_host(c, nz, zones->zone_ptr);
+  begin_call(c);
+  if (c->timing) c->timer.mark(PH_PREP);
+  hipEvent_t e0, e1;
+  check(c, hipEventCreate(&e0), "event");
+  check(c, hipEventCreate(&e1), "event");
+  check(c, hipEventRecord(e0, c->stream), "event");
+  Geo g = stage_geo(c, zones, obs, dev, true);
+  int64_t ld_hs = 0;
+  const double* dHSf = stage2d(c, Workspace::IN_HSF, HSf, m, N, ldHSf, dev, &ld_hs);
+  const double* dHxf = stage(c, Workspace::IN_HXF, Hxf, m, dev);
+  double* HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+  double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+  double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+  double* zbuf = (double*)c->ws.get(Workspace::NLL_Z, sizeof(double) * (3 * std::max<int64_t>(nz, 1) + 2));
+  double* gth = dev ? grad_log_sigma
+                    : (double*)c->ws.get(Workspace::NLL_GTH, sizeof(double) * std::max<int64_t>(m, 1));
+  if (!HS || !innov || !rho || !zbuf || !gth) check(c, hipErrorOutOfMemory, "workspace");
+  double* Jz = zbuf;
+  double* gLz = zbuf + nz;
+  double* gAz = zbuf + 2 * nz;
+  double* tot = zbuf + 3 * nz;
+  fill_f64(gth, m, 0.0, c->stream);
+  PrepLocArgs P{m, N, dHSf, ld_hs, dHxf, g.od.yo, g.od.rmse2, g.od.excluded, HS, innov, rho,
+                c->d_err, inflation};
+  launch_prep_loc(NP, P, c->stream);
+  check(c, hipGetLastError(), "prep");
+  if (c->timing) c->timer.mark(PH_SELECT);
+  SelDev sel{};
+  int64_t total = 0;
+  check(c, run_selection(c->ws, g.zd, g.od, 1, c->stream, c->d_err, sel, &total), "selection");
+  int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+  if (!sweeps) check(c, hipErrorOutOfMemory, "workspace");
+  ChunkArgs A{};
+  A.N = N;
+  A.mode = 2;
+  A.m = m;
+  A.zone_ptr = g.zd.zone_ptr;
+  A.sel_ptr = sel.ptr;
+  A.sel_idx = sel.idx;
+  A.sel_w = sel.w;
+  A.rho = rho;
+  A.innov = innov;
+  A.HS = HS;
+  A.err = c->d_err;
+  A.tol = c->jacobi_tol > 0 ? c->jacobi_tol : 4.0 * NP * 2.220446049250313e-16;
+  A.max_sweeps = c->max_sweeps;
+  A.infl = inflation;
+  A.corr = g.zd.corr;
+  A.nll_J = Jz;
+  A.nll_gL = gLz;
+  A.nll_gA = gAz;
+  A.nll_gth = gth;
+  run_zones(c, NP, A, nz, sweeps);
+  launch_nll_totals(nz, Jz, gAz, tot, c->stream);
+  check(c, hipGetLastError(), "totals");
+  if (c->timing) c->timer.mark(PH_END);
+  check(c, hipEventRecord(e1, c->stream), "event");
+  finish_call(c, nz, sel.ptr, sweeps);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  collect_stats(c, nz, sel.ptr, ms);
+  const hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  check(c, hipMemcpy(nll, tot, sizeof(double), k), "copy nll");
+  check(c, hipMemcpy(grad_inflation, tot + 1, sizeof(double), k), "copy grad");
+  if (nz > 0) check(c, hipMemcpy(grad_corr_len, gLz, sizeof(double) * nz, k), "copy grad");
+  if (nll_zone && nz > 0) check(c, hipMemcpy(nll_zone, Jz, sizeof(double) * nz, k), "copy nll");
+  if (!dev && m > 0)
+    check(c, hipMemcpy(grad_log_sigma, gth, sizeof(double) * m, hipMemcpyDeviceToHost), "D2H");
+  (void)total;
   return OAKG_OK;
 }
 
@@ -797,6 +894,17 @@ int oakg_ensemble_analysis_f32(oakg_ctx* c, const oakg_zones* zones, const oakg_
                                double max_correction, uint32_t flags) {
   return ensemble_impl<float>(c, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, inflation,
                               max_correction, flags);
+}
+
+int oakg_local_nll(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, int32_t N,
+                   const double* Hxf, const double* HSf, int64_t ldHSf, double inflation,
+                   double* nll, double* grad_log_sigma, double* grad_corr_len,
+                   double* grad_inflation, double* nll_zone, uint32_t flags) {
+  return guarded(c, [&] {
+    if (!zones || !obs) arg_fail(c, "zones and obs are required");
+    return local_nll_impl(c, zones, obs, N, Hxf, HSf, ldHSf, inflation, nll, grad_log_sigma,
+                          grad_corr_len, grad_inflation, nll_zone, flags);
+  });
 }
 
 int oakg_coo_to_csr(int64_t m, int64_t nnz, const int32_t* row, const int32_t* col,

@@ -15,7 +15,8 @@
 //                  the rows of Q in LDS, giving the eigenvectors U = Q Z.
 //   k_eigvec_epi   per zone: a = U diag(1/mu) U^T b (+ one refinement step
 //                  against M), v ~ U diag(mu^1/2) U^T 1, Y = U diag(mu^-1/4),
-//                  with mu = max(lambda, 1) (rrsqrt.F90:137-142, 176-177).
+//                  with mu = max(lambda, 1) (rrsqrt.F90:137-142, 176-177);
+//                  mu itself is kept beside a and v for the likelihood.
 #include "analysis.hpp"
 #include "common.hpp"
 
@@ -572,6 +573,7 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, int N, const int
   if (hh == 0) {
     V[NP + j] = ar;
     V[2 * NP + j] = vn;
+    V[3 * NP + j] = muc;
     if (ampl && j < N) ampl[(size_t)z * N + j] = ar;
     if (ar != ar) atomicOr(err, OAKG_DERR_NAN);
   }

@@ -336,3 +336,38 @@ def config2_gpu(N=128, device="cuda:0", dtype=None):
     return column_ocean_gpu(360, 180, 30, N, 2, n_prof_cols=60,
                             ssh_density=lognormal_density(145.0, 1.2, 18.0), dtype=dtype,
                             device=device)
+
+
+def config3(nx=256, N=32, m=4000, seed_cfg=3):
+    """Differentiable-analysis configuration: the config 0 layout on an
+    nx x nx grid (n = nx^2, one zone per point), N GRF members (l = 8 cells),
+    m observations uniform in [1, nx)^2 with bilinear H, taper L = 5, cutoff 15.
+    The observation errors are heterogeneous (sigma = 0.05 for even, 0.2 for
+    odd observation indices): a per-observation log sigma is worth tuning."""
+    g = rng(seed_cfg)
+    ny = nx
+    n = nx * ny
+    fields = grf(g, (nx, ny), (8, 8), N + 1)
+    E = np.asfortranarray(np.stack([f.ravel(order="F") for f in fields[:N]], 1))
+    truth = fields[N].ravel(order="F")
+    x = g.uniform(1, nx, m)
+    y = g.uniform(1, ny, m)
+    i0, j0, w = bilinear(x, y, nx, ny)
+    Hi = np.repeat(np.arange(m), 4)
+    cols = []
+    for di, dj in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        cols.append((i0 + di - 1) + nx * (j0 + dj - 1))
+    Hj = np.stack(cols, 1).ravel()
+    Hs = w.ravel()
+    sigma = np.where(np.arange(m) % 2 == 0, 0.05, 0.2)
+    Hd_truth = np.zeros(m)
+    np.add.at(Hd_truth, Hi, Hs * truth[Hj])
+    yo = Hd_truth + sigma * g.standard_normal(m)
+    gi, gj = np.meshgrid(np.arange(1, nx + 1, dtype=float), np.arange(1, ny + 1, dtype=float),
+                         indexing="ij")
+    zpos = np.zeros((n, 4))
+    zpos[:, 0] = gi.ravel(order="F")
+    zpos[:, 1] = gj.ravel(order="F")
+    return Problem("cfg3", n, N, m, np.arange(n + 1, dtype=np.int64), zpos, np.full(n, 5.0),
+                   np.full(n, 15.0), E, Hi.astype(np.int32), Hj.astype(np.int32), Hs,
+                   np.zeros(m), x, y, yo, sigma ** 2, truth=truth)

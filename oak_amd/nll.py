@@ -162,7 +162,7 @@ def tune(prob, log_sigma0, L0, infl0, steps=50, lr=0.05, ctx=None, callback=None
         opt.zero_grad()
         J = innovation_nll(ls, lL, la, prob, ctx)
         J.backward()
-        hist.append(float(J))
+        hist.append(float(J.detach()))
         if callback:
             callback(it, float(J), ls, lL, la)
         opt.step()

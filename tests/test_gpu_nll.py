@@ -26,7 +26,7 @@ def _ref(P, prob, ls, L, a):
     ar = torch.tensor(a, dtype=torch.float64, requires_grad=True)
     J = nll_reference(sel, prob.HSf, prob.Hxf, prob.yo, lsr, Lr, ar)
     J.backward()
-    return float(J), lsr.grad.numpy(), float(Lr.grad), float(ar.grad)
+    return float(J.detach()), lsr.grad.numpy(), float(Lr.grad), float(ar.grad)
 
 
 @pytest.mark.parametrize("L,a", [(5.0, 1.0), (3.0, 1.3)])

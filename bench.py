@@ -80,7 +80,9 @@ def flops_model(N, counts, zone_sizes):
 
 def cpu_baseline(DP, target_s=15.0):
     """Oracle (LAPACK backend, OpenMP dynamic over zones, OAK's brute-force
-    selection) on a contiguous slice of zones in the middle of the domain."""
+    selection) on a strided sample of zones spread over the whole domain (every
+    k-th zone, so that dense and sparse observation regions are represented in
+    proportion); the sampled zones' state rows are gathered into one slice."""
     import torch
     from oracle import oracle as O
     P = DP.host
@@ -99,29 +101,36 @@ def cpu_baseline(DP, target_s=15.0):
     HSf = ((HE - Hxf[:, None]) / np.sqrt(N - 1)).cpu().numpy()
     Hxf = Hxf.cpu().numpy()
     nz = P.nzones
-    zc = nz // 2
+    zs = np.diff(P.zone_ptr)
 
-    def run(nzs):
-        z0, z1 = zc, min(nz, zc + nzs)
-        i0, i1 = int(P.zone_ptr[z0]), int(P.zone_ptr[z1])
-        Ez = E[i0:i1, :].double().cpu().numpy()
+    def run(stride):
+        sel = np.arange(stride // 2, nz, stride)
+        starts = P.zone_ptr[sel]
+        rows_ = np.concatenate([np.arange(s, s + c) for s, c in zip(starts, zs[sel])])
+        Ez = E[torch.as_tensor(rows_, device=E.device), :].double().cpu().numpy()
         xf = Ez.mean(axis=1)
         Sf = np.asfortranarray((Ez - xf[:, None]) / np.sqrt(N - 1))
-        geo = dict(zone_ptr=P.zone_ptr[z0:z1 + 1] - i0, zpos=P.zpos[z0:z1], L=P.corr_len[z0:z1],
-                   cutoff=P.max_len[z0:z1], metric=P.metric, loctype=P.loctype, ox=P.ox, oy=P.oy)
+        zp = np.concatenate([[0], np.cumsum(zs[sel])]).astype(np.int64)
+        geo = dict(zone_ptr=zp, zpos=np.ascontiguousarray(P.zpos[sel]),
+                   L=np.ascontiguousarray(P.corr_len[sel]),
+                   cutoff=np.ascontiguousarray(P.max_len[sel]), metric=P.metric,
+                   loctype=P.loctype, ox=P.ox, oy=P.oy)
         t0 = time.perf_counter()
         O.loc_analysis_geo(geo, xf, Hxf, P.yo, P.rmse2, Sf, HSf, nthreads=cores)
-        return time.perf_counter() - t0, i1 - i0, z1 - z0
+        return time.perf_counter() - t0, len(rows_), len(sel)
 
-    t, ne, nzs = run(max(cores, 16))
+    probe_stride = max(1, nz // max(4 * cores, 64))
+    t, ne, nzs = run(probe_stride)
     per_zone = t / nzs
     print("# cpu baseline probe: %d zones in %.2f s" % (nzs, t), file=sys.stderr, flush=True)
-    nzs2 = int(min(nz - zc, max(cores, target_s / max(per_zone, 1e-6))))
-    t, ne, nzs = run(nzs2)
+    want = max(4 * cores, int(target_s / max(per_zone, 1e-6)))
+    stride = max(1, nz // want)
+    t, ne, nzs = run(stride)
     return {"value": ne / t, "unit": "analysed state elems/s", "cores": cores, "kind": "port",
-            "sample": "%d contiguous zones (%d state elements) of the same workload, zone loop "
-                      "of locAnalysis with OAK's brute-force selection; %s eigensolver, %.1f s"
-                      % (nzs, ne, "LAPACK dsyev (OpenBLAS)" if lapack else "Jacobi", t)}
+            "sample": "%d zones (every %d-th of %d; %d state elements) of the same workload, "
+                      "zone loop of locAnalysis with OAK's brute-force selection; %s "
+                      "eigensolver, %.1f s" % (nzs, stride, nz, ne,
+                                               "LAPACK dsyev (OpenBLAS)" if lapack else "Jacobi", t)}
 
 
 PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
@@ -152,7 +161,7 @@ def run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea, label):
     return ms, phase, st
 
 
-def roofline(F, phase, mixed, eig_name):
+def roofline(F, phase, mixed, eig_name, config):
     """Dominant phase of the step and its achieved rate against the MI355X peak
     of the arithmetic it runs on (FP32 MFMA for the mixed contractions, FP64
     otherwise)."""
@@ -169,7 +178,26 @@ def roofline(F, phase, mixed, eig_name):
     peak = PEAK_FP32_MFMA if (mixed and kern in ("gram", "update")) else PEAK_FP64
     ach = work[kern] / (phase[kern] * 1e-3)
     return {"bound": "mfma", "kernel": names[kern], "achieved": ach / 1e12, "peak": peak / 1e12,
-            "unit": "TFLOP/s", "frac": ach / peak, "traffic": None}
+            "unit": "TFLOP/s", "frac": ach / peak, "traffic": phase_traffic(kern, mixed, config)}
+
+
+PHASE_KERNELS = {"eig": ["k_tridiag", "k_tql", "k_tql_apply", "k_eigvec_epi"],
+                 "gram": ["k_gram128"], "tmat": ["k_tmat128"]}
+
+
+def phase_traffic(kern, mixed, config):
+    """HBM bytes moved by the phase's kernels in one analysis, from the
+    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed under
+    profiles/ (config 2 only; None otherwise)."""
+    path = os.path.join(ROOT, "profiles", "traffic_config%d.json" % config)
+    if not os.path.exists(path):
+        return None
+    tab = json.load(open(path))
+    names = PHASE_KERNELS.get(kern) or (["k_update_f32"] if mixed else ["k_update128"])
+    per = tab["bytes_per_dispatch"]
+    if not all(k in per for k in names):
+        return None
+    return float(sum(per[k]["read"] + per[k]["write"] for k in names) * tab["chunks_per_call"])
 
 
 def run_config3(args, device):
@@ -325,7 +353,7 @@ def main():
                    "mean_p": float(counts.mean()), "median_p": float(np.median(counts)),
                    "max_p": int(counts.max()),
                    "phase_ms": {k: round(v, 3) for k, v in phase.items()}},
-        "roofline": roofline(F, phase, head == "mixed", eig_name),
+        "roofline": roofline(F, phase, head == "mixed", eig_name, args.config),
     }
     if "mixed" in res and "fp64" in res:
         ms64, ph64, _ = res["fp64"]

@@ -1,0 +1,62 @@
+! Fortran driver of locAnalysis_gpu: a small 1-D local analysis with a
+! Gaussian taper callback (OAK's selectObservations interface).  Writes the
+! inputs, the outputs and the return code to a binary file for the tests.
+!
+!   oakg_fdriver <output file>
+module fdriver_geom
+  implicit none
+  integer, parameter :: nzone = 10, zsize = 2, nmem = 6, nobs = 8
+contains
+  subroutine select_gauss(i, c, relevantObs)
+    integer, intent(in) :: i
+    real(8), intent(out) :: c(:)
+    logical, intent(out) :: relevantObs(:)
+    integer :: l
+    real(8) :: d
+    do l = 1, size(c)
+      d = abs(dble(i) - 1.25d0 * dble(l))
+      relevantObs(l) = d <= 5.0d0
+      c(l) = exp(-(d / 2.0d0)**2)
+    end do
+  end subroutine select_gauss
+end module fdriver_geom
+
+program oakg_fdriver
+  use iso_c_binding
+  use oakg_rrsqrt
+  use fdriver_geom
+  implicit none
+  integer :: zoneSize(nzone), i, k, l, info, u, n
+  real(c_double), allocatable, target :: xf(:), Hxf(:), yo(:), Sf(:,:), HSf(:,:), Rd(:), xa(:), Sa(:,:)
+  character(len=1024) :: fname
+
+  call get_command_argument(1, fname)
+  n = nzone * zsize
+  zoneSize = zsize
+  allocate(xf(n), Hxf(nobs), yo(nobs), Sf(n, nmem), HSf(nobs, nmem), Rd(nobs), xa(n), Sa(n, nmem))
+  do i = 1, n
+    xf(i) = sin(0.3d0 * i)
+    do k = 1, nmem
+      Sf(i, k) = cos(0.7d0 * i * k) / nmem
+    end do
+  end do
+  do l = 1, nobs
+    yo(l) = 0.1d0 * l
+    Hxf(l) = 0.005d0 * l * l
+    Rd(l) = 0.1d0 + 0.01d0 * l
+    do k = 1, nmem
+      HSf(l, k) = cos(0.5d0 * l + 0.9d0 * k) / nmem
+    end do
+  end do
+  xa = 0
+  Sa = 0
+  call locAnalysis_gpu(zoneSize, select_gauss, xf, Hxf, yo, Sf, HSf, Rd, xa, Sa, info=info)
+  print '(a,i0)', 'locAnalysis_gpu info = ', info
+  if (len_trim(fname) > 0) then
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(u) int(info, 8), int(nzone, 8), int(zsize, 8), int(nmem, 8), int(nobs, 8)
+    write(u) xf, Hxf, yo, Sf, HSf, Rd, xa, Sa
+    close(u)
+  end if
+  if (info /= 0) call exit(info)
+end program oakg_fdriver

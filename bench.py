@@ -166,23 +166,31 @@ def roofline(F, phase, mixed, eig_name, config):
     of the arithmetic it runs on (FP32 MFMA for the mixed contractions, FP64
     otherwise)."""
     work = {"gram": F["gram"], "eig": F["eig"], "tmat": F["trans"], "update": F["update"]}
+    # N > 64: transform and update run in one kernel, booked as the update
+    fused = phase["tmat"] == 0.0
+    if fused:
+        work["update"] += work["tmat"]
     kern = max(work, key=lambda k: phase[k])
     names = {
-        "gram": ("k_gram_f32 (S^T R^-1 S, v_mfma_f32_16x16x4f32)" if mixed else
-                 "k_gram (S^T R^-1 S, v_mfma_f64_16x16x4f64)"),
+        "gram": ("k_gram128<float> (S^T R^-1 S from FP32 S, v_mfma_f64_16x16x4f64)" if mixed else
+                 "k_gram128 (S^T R^-1 S, v_mfma_f64_16x16x4f64)"),
         "eig": eig_name,
         "tmat": "k_tmat (T = Y Y^T Omega, v_mfma_f64_16x16x4f64)",
         "update": ("k_update_f32 (Ea = xa + A' W, v_mfma_f32_16x16x4f32)" if mixed else
                    "k_update (Ea = xa + A' W, v_mfma_f64_16x16x4f64)"),
     }
-    peak = PEAK_FP32_MFMA if (mixed and kern in ("gram", "update")) else PEAK_FP64
+    if fused:
+        names["update"] = ("k_xform128 (T = Y Y^T Omega in LDS on v_mfma_f64_16x16x4f64, then "
+                           "Ea = xa + A' T on %s)" % ("v_mfma_f32_16x16x4f32" if mixed else
+                                                     "v_mfma_f64_16x16x4f64"))
+    peak = PEAK_FP32_MFMA if (mixed and kern == "update" and not fused) else PEAK_FP64
     ach = work[kern] / (phase[kern] * 1e-3)
     return {"bound": "mfma", "kernel": names[kern], "achieved": ach / 1e12, "peak": peak / 1e12,
             "unit": "TFLOP/s", "frac": ach / peak, "traffic": phase_traffic(kern, mixed, config)}
 
 
 PHASE_KERNELS = {"eig": ["k_tridiag", "k_tql", "k_tql_apply", "k_eigvec_epi"],
-                 "gram": ["k_gram128"], "tmat": ["k_tmat128"]}
+                 "gram": ["k_gram128"], "update": ["k_xform128"]}
 
 
 def phase_traffic(kern, mixed, config):
@@ -193,7 +201,9 @@ def phase_traffic(kern, mixed, config):
     if not os.path.exists(path):
         return None
     tab = json.load(open(path))
-    names = PHASE_KERNELS.get(kern) or (["k_update_f32"] if mixed else ["k_update128"])
+    names = PHASE_KERNELS.get(kern)
+    if not names:
+        return None
     per = tab["bytes_per_dispatch"]
     if not all(k in per for k in names):
         return None

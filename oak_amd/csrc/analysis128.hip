@@ -10,8 +10,10 @@
 //              runs the 120 intra-block pairs of its blocks, then 7 block
 //              rounds (round-robin over the 8 blocks) of 16 cross-block rounds;
 //              blocks move between waves through the zone's scratch matrix
-//   k_tmat128  T = (Y Y^T) Omega with Y Y^T on MFMA (operands from L2)
-//   k_update   as for NP <= 64, with [T | a] read from global memory
+//   k_xform128 T = (Y Y^T) Omega formed in LDS and the update Ea = xa + Sf T
+//              run from it (FP64 MFMA; FP32 MFMA for FP32 members)
+#include <type_traits>
+
 #include "analysis.hpp"
 #include "common.hpp"
 
@@ -439,151 +441,325 @@ __global__ __launch_bounds__(256) void k_eig128(int64_t z0, int nzc, int N,
 }
 
 // ---------------------------------------------------------------------------
-// T = (Y Y^T) Omega, NP = 128 (Omega as in analysis.hip's k_tmat)
+// Transform and update fused, NP = 128, FP64: T = (Y Y^T) Omega is formed in
+// LDS and out = Sf_z T, xa = x0 + Sf_z a run from it, so T never goes through
+// HBM.  One workgroup per zone (the 128 x 128 block fills the LDS):
+//   1. Y -> LDS (row stride LDT: the MFMA operand reads, 16 rows x 4
+//      consecutive doubles per instruction, hit distinct banks)
+//   2. Y Y^T on v_mfma_f64_16x16x4f64, the 36 lower tiles (wave w owns tile
+//      rows w and 7 - w, 9 tiles each), stored mirrored over Y
+//   3. the Omega correction (the rank-structured formula of k_tmat in
+//      analysis.hip), applied in place; the buffer then holds X[s][r] = T[r][s]
+//   4. the update (Out^T strip = X Sf_strip^T) with the A operand X read
+//      from LDS and each wave's first strip of Sf loaded during steps 1-3
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_tmat128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
-                                                 const double* __restrict__ ymat,
-                                                 double* __restrict__ mat,
-                                                 const double* __restrict__ vec) {
-  __shared__ double U[4][P128];
-  const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
-  double* T = mat + (size_t)zl * P128 * P128;
-  if (sel_ptr[z + 1] == sel_ptr[z]) {
-    for (int e = threadIdx.x; e < P128 * P128; e += 256) T[e] = (e / P128 == e % P128) ? 1.0 : 0.0;
-    return;
-  }
-  const double* Y = ymat + (size_t)zl * P128 * P128;
-  const double* v = vec + (size_t)zl * 4 * P128 + 2 * P128;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = lane & 15, kk = lane >> 4;
-  // wave w: tile rows I = 2w, 2w+1, all tile columns
-  d4 acc[2][8];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int J = 0; J < 8; ++J) acc[a][J] = d4{0, 0, 0, 0};
-  for (int j0 = 0; j0 < P128; j0 += 4) {
-    double bf[8];
-#pragma unroll
-    for (int J = 0; J < 8; ++J) bf[J] = Y[(16 * J + li) * P128 + j0 + kk];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const double af = bf[2 * wave + a];
-#pragma unroll
-      for (int J = 0; J < 8; ++J) acc[a][J] = mfma_f64(af, bf[J], acc[a][J]);
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int J = 0; J < 8; ++J)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        T[(16 * (2 * wave + a) + (lane >> 4) + 4 * r) * P128 + 16 * J + li] = acc[a][J][r];
-  __syncthreads();
-  const double wN = 1.0 / sqrt((double)N);
-  const double vN = v[N - 1];
-  const double sgv = vN >= 0 ? 1.0 : -1.0;
-  const double alv = -1.0 / (fabs(vN) + 1.0);
-  const double alw = -1.0 / (wN + 1.0);
-  double vw = 0;
-  for (int s = 0; s < N - 1; ++s) vw = fma(v[s], wN, vw);
-  if (threadIdx.x < P128) {
-    const int r = threadIdx.x;
-    double u1 = 0, u2 = 0, u3 = 0;
-    for (int q = 0; q < P128; ++q) {
-      const double t = T[r * P128 + q];
-      const double vq = q < N ? v[q] : 0.0;
-      u1 = fma(t, vq, u1);
-      u2 = fma(t, q < N - 1 ? wN : 0.0, u2);
-      u3 = fma(t, vq + (q == N - 1 ? sgv : 0.0), u3);
-    }
-    U[0][r] = T[r * P128 + N - 1];
-    U[1][r] = u1;
-    U[2][r] = u2;
-    U[3][r] = u3;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < P128 * P128; e += 256) {
-    const int r = e / P128, s = e % P128;
-    if (s < N) {
-      double t = T[e];
-      const double zw = wN + (s == N - 1 ? 1.0 : 0.0);
-      const double vt = (s < N - 1) ? v[s] : 0.0;
-      if (s == N - 1) t -= U[0][r];
-      t += U[1][r] * wN + alw * U[2][r] * zw + alv * U[3][r] * vt + alv * alw * vw * U[3][r] * zw;
-      T[e] = t;
-    }
-  }
-}
+constexpr int LDT = P128 + 2;
 
-// ---------------------------------------------------------------------------
-// Update for NP = 128: out = Sf_z [T | a], [T | a] read from global memory.
-// ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(256) void k_update128(int64_t z0, int N, const int64_t* __restrict__ zone_ptr,
-                                                   const double* __restrict__ mat,
-                                                   const double* __restrict__ vec,
-                                                   const double* __restrict__ X0,
-                                                   const double* __restrict__ S, int64_t ldS,
-                                                   double* __restrict__ xa, double* __restrict__ Out,
-                                                   int64_t ldO, double scaling, double infl,
-                                                   double maxcorr) {
+struct XformLds {
+  double t[P128 * LDT];
+  double u[4][P128];   // quarter sums of T v
+  double w[4][P128];   // quarter sums of T over the first N-1 entries
+  double v[P128];
+  double a[P128 + 2];  // a, then v.w~
+};
+
+template <int MODE, typename TE>
+__global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+                                                  const int64_t* __restrict__ zone_ptr,
+                                                  const double* __restrict__ ymat,
+                                                  const double* __restrict__ vec,
+                                                  const double* __restrict__ X0,
+                                                  const TE* __restrict__ S, int64_t ldS,
+                                                  double* __restrict__ xa, TE* __restrict__ Out,
+                                                  int64_t ldO, double scaling, double infl,
+                                                  double maxcorr) {
 #pragma clang fp contract(off)
-  constexpr int NB = 8;
+  // TE = double: FP64 state, update on FP64 MFMA (MODE 0: locAnalysis with
+  // Sf given, MODE 1: ensemble).  TE = float: FP32 members (MODE 1 only),
+  // update on FP32 MFMA with X rounded to FP32, Sf a and the mean in FP64.
+  constexpr bool F32 = std::is_same<TE, float>::value;
+  static_assert(!F32 || MODE == 1, "FP32 state is the ensemble mode");
+  __shared__ XformLds L;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, kk = lane >> 4;
   const int64_t i0 = zone_ptr[z];
   const int64_t sz = zone_ptr[z + 1] - i0;
-  const double* T = mat + (size_t)zl * P128 * P128;
-  const double* a = vec + (size_t)zl * 4 * P128 + P128;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = lane & 15, kk = lane >> 4;
-  for (int64_t rb = 16 * wave; rb < sz; rb += 64) {
-    d4 acc[NB + 1];
-#pragma unroll
-    for (int J = 0; J <= NB; ++J) acc[J] = d4{0, 0, 0, 0};
+  // The k index of the MFMA steps is permuted within pairs of steps: in
+  // steps 2p and 2p+1 lane group kk covers k = 8p + 2kk and 8p + 2kk + 1, so
+  // one 16-byte LDS read gives a lane both A operands.  B uses the same map.
+  // Sf values of strip rb: b[q] = Sf[rb + li][k(q)], k(q) = 8 (q/2) + 2 kk + q%2,
+  // loaded raw from clamped addresses; the selection and centring are
+  // applied where the values are first used, so the loads stay in flight
+  TE b[P128 / 4];
+  double x0 = 0.0;
+  auto load_strip = [&](int64_t rb) {
     const int64_t row = rb + li;
-    const bool rok = row < sz;
-    const double xb = (MODE == 1 && rok) ? X0[i0 + row] : 0.0;
-    for (int k0 = 0; k0 < P128; k0 += 4) {
-      const int kc = k0 + kk;
-      double bval = 0.0;
-      if (rok && kc < N) {
-        const double sv = S[(i0 + row) + (int64_t)kc * ldS];
-        bval = (MODE == 1) ? (sv - xb) / scaling : sv;
-      }
+    const int64_t rr = i0 + (row < sz ? row : 0);
+    x0 = X0[rr];
 #pragma unroll
-      for (int J = 0; J < NB; ++J) acc[J] = mfma_f64(T[kc * P128 + 16 * J + li], bval, acc[J]);
-      acc[NB] = mfma_f64(li == 0 ? a[kc] : 0.0, bval, acc[NB]);
+    for (int q = 0; q < P128 / 4; ++q) {
+      const int k = 8 * (q >> 1) + 2 * kk + (q & 1);
+      b[q] = S[rr + (int64_t)(k < N ? k : 0) * ldS];
     }
-    const double dx = shfl_d(acc[NB][0], li);
-    double xar = 0.0;
-    if (rok) {
-      const double x0 = X0[i0 + row];
-      xar = x0 + dx;
-      if (MODE == 1) {
-        if (xar - maxcorr > x0) xar = x0 + maxcorr;
-        if (xar < x0 - maxcorr) xar = x0 - maxcorr;
-      }
-      if (MODE == 0 && kk == 0) xa[i0 + row] = xar;
+  };
+  const double* Vz = vec + (size_t)zl * 4 * P128;
+  if (tid < P128) L.a[tid] = F32 ? Vz[P128 + tid] / scaling : Vz[P128 + tid];
+  else if (tid < 2 * P128) L.v[tid - P128] = tid - P128 < N ? Vz[2 * P128 + tid - P128] : 0.0;
+  if (sel_ptr[z + 1] == sel_ptr[z]) {
+    // no observations: T = I (a = 0), the update copies the forecast anomalies
+    for (int e = tid; e < P128 * P128; e += 512) L.t[(e >> 7) * LDT + (e & 127)] = (e >> 7) == (e & 127);
+    if (sz > 0) load_strip(16 * wave);
+    lds_barrier();
+  } else {
+    // Y is stored to LDS first, then the wave's first strip of Sf is
+    // loaded; the barriers of this kernel order LDS only, so the Sf loads
+    // stay in flight while the transform is formed
+    const double2* Y2 = reinterpret_cast<const double2*>(ymat + (size_t)zl * P128 * P128);
+#pragma unroll
+    for (int it = 0; it < P128 * P128 / 1024; ++it) {
+      const int e = tid + 512 * it;
+      *reinterpret_cast<double2*>(L.t + (e >> 6) * LDT + 2 * (e & 63)) = Y2[e];
     }
+    __builtin_amdgcn_sched_barrier(0);
+    if (sz > 0) load_strip(16 * wave);
+    lds_barrier();
+    {
+      // 36 lower tiles over 8 waves: the pair of tile rows R0 = w % 4 and
+      // R1 = 7 - w % 4 has 9 tiles; wave w < 4 takes (R0, 0..R0) and
+      // (R1, 0..3-R0), wave w + 4 takes (R1, 4-R0..7-R0).
+      const int R0 = wave & 3, R1 = 7 - R0;
+      const bool hi = wave >= 4;
+      const int jlo = hi ? 4 - R0 : 0, jhi = hi ? 7 - R0 : 3 - R0;  // row R1 range
+      d4 acc0[4], acc1[4];
 #pragma unroll
-    for (int J = 0; J < NB; ++J) {
+      for (int J = 0; J < 4; ++J) acc0[J] = acc1[J] = d4{0, 0, 0, 0};
+      const double* yr = L.t + li * LDT + 2 * kk;
+      for (int p = 0; p < P128 / 8; ++p) {
+        asm volatile("" ::: "memory");
+        const double2 y0 = *reinterpret_cast<const double2*>(yr + 16 * R0 * LDT + 8 * p);
+        const double2 y1 = *reinterpret_cast<const double2*>(yr + 16 * R1 * LDT + 8 * p);
+        // tile columns jlo + J (for the waves w < 4, jlo = 0: also the columns of row R0)
+        double2 yb[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int col = 16 * J + kk + 4 * r;
-        if (rok && col < N) {
-          double o = acc[J][r];
-          if (MODE == 1) {
-            double sa = (infl != 1.0) ? infl * o : o;
-            o = xar + sa * scaling;
+        for (int J = 0; J < 4; ++J)
+          yb[J] = *reinterpret_cast<const double2*>(yr + 16 * (jlo + J) * LDT + 8 * p);
+        if (!hi) {
+#pragma unroll
+          for (int J = 0; J < 4; ++J)
+            if (J <= R0) {
+              acc0[J] = mfma_f64(y0.x, yb[J].x, acc0[J]);
+              acc0[J] = mfma_f64(y0.y, yb[J].y, acc0[J]);
+            }
+        }
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+          if (jlo + J <= jhi) {
+            acc1[J] = mfma_f64(y1.x, yb[J].x, acc1[J]);
+            acc1[J] = mfma_f64(y1.y, yb[J].y, acc1[J]);
           }
-          Out[(i0 + row) + (int64_t)col * ldO] = o;
+      }
+      lds_barrier();  // every wave has read Y
+      // D[16 I + lane / 16 + 4 r][16 J + lane % 16]; mirrored entries keep T symmetric
+      auto put = [&](int I, int J, const d4& acc) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = 16 * I + kk + 4 * r, n = 16 * J + li;
+          L.t[m * LDT + n] = acc[r];
+          L.t[n * LDT + m] = acc[r];
+        }
+      };
+      if (!hi) {
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+          if (J <= R0) put(R0, J, acc0[J]);
+      }
+#pragma unroll
+      for (int J = 0; J < 4; ++J)
+        if (jlo + J <= jhi) put(R1, jlo + J, acc1[J]);
+    }
+    lds_barrier();
+    // Omega = v w^T + P(v) P(w)^T (rrsqrt.F90:737-756, matoper.F90:509-535)
+    // applied on the right of the symmetric T (as k_tmat in analysis.hip).
+    // With w = 1/sqrt(N) 1,
+    // alpha_v = -1/(|v_N| + 1), alpha_w = -1/(1/sqrt(N) + 1), row r of T Omega is
+    //   x[s] = T[r][s] - [s = N-1] T[r][N-1] + wN (T v)_r
+    //          + (alpha_w U2 + alpha_v alpha_w (v.w~) U3) zw[s] + alpha_v U3 vt[s]
+    // with U2 = wN sum_{q<N-1} T[r][q], U3 = (T v)_r + sign(v_N) T[r][N-1],
+    // zw[s] = wN + [s = N-1], vt[s] = v[s] for s < N-1 (0 at N-1); the row
+    // sums are read as column sums of the symmetric T.
+    const double wN = 1.0 / sqrt((double)N);
+    const double vN = L.v[N - 1];
+    const double sgv = vN >= 0 ? 1.0 : -1.0;
+    const double alv = -1.0 / (fabs(vN) + 1.0);
+    const double alw = -1.0 / (wN + 1.0);
+    {
+      const int r = tid & 127, h = tid >> 7;  // quarter h of the column
+      double su = 0, sw = 0;
+#pragma unroll 8
+      for (int q = 32 * h; q < 32 * h + 32; ++q) {
+        const double tq = L.t[q * LDT + r];
+        su = fma(tq, L.v[q], su);
+        sw += q < N - 1 ? tq : 0.0;
+      }
+      L.u[h][r] = su;
+      L.w[h][r] = sw;
+    }
+    if (wave == 0) {
+      double s = 0;
+      for (int q = lane; q < N - 1; q += 64) s = fma(L.v[q], wN, s);
+      s = wave_sum_fast(s);
+      if (lane == 0) L.a[P128] = s;
+    }
+    lds_barrier();
+    const int r = tid & 127;
+    const double u0 = L.t[(N - 1) * LDT + r];
+    const double u1 = (L.u[0][r] + L.u[1][r]) + (L.u[2][r] + L.u[3][r]);
+    const double u2 = wN * ((L.w[0][r] + L.w[1][r]) + (L.w[2][r] + L.w[3][r]));
+    const double u3 = u1 + sgv * u0;
+    const double vw = L.a[P128];
+    const double c1 = u1 * wN;
+    const double c2 = alw * u2 + alv * alw * vw * u3;
+    const double c3 = alv * u3;
+    // X[s][r] = (T Omega)[r][s], in place (T symmetric: T[r][s] = t[s][r])
+#pragma unroll 4
+    for (int s = tid >> 7; s < N; s += 4) {
+      const double zw = s == N - 1 ? wN + 1.0 : wN;
+      const double vt = s < N - 1 ? L.v[s] : 0.0;
+      double x = L.t[s * LDT + r];
+      if (s == N - 1) x -= u0;
+      x += c1 + c2 * zw + c3 * vt;
+      L.t[s * LDT + r] = x;
+    }
+    lds_barrier();
+  }
+  // out = Sf_z T: Out^T strip = T^T Sf_strip^T, A operand X[16 J + li][k]
+  const double* xr = L.t + li * LDT + 2 * kk;
+  if constexpr (F32) {
+    // mixed precision (the arithmetic of k_update_f32): b = (E - xbar)/sqrt(N-1)
+    // in FP32, xa = xbar + clamp(sum_k (E_k - xbar) a_k/sqrt(N-1)) in FP64,
+    // Ea = xa + sqrt(N-1) infl (Sf T) with the f32 MFMA D layout
+    // D[4 (lane / 16) + r][lane % 16]
+    const double sc_infl = scaling * infl;
+    const float inv = (float)(1.0 / scaling);
+    for (int64_t rb = 16 * wave; rb < sz; rb += 128) {
+      if (rb != 16 * wave) load_strip(rb);
+      const int64_t row = rb + li;
+      const bool rok = row < sz;
+      const float xb = (float)x0;
+      float(&bf)[P128 / 4] = b;  // centred in place
+      double dxp = 0.0;
+#pragma unroll
+      for (int q = 0; q < P128 / 4; ++q) {
+        const int kc = 8 * (q >> 1) + 2 * kk + (q & 1);
+        const bool ok = rok && kc < N;
+        const float ev = b[q];
+        if (ok) dxp = fma((double)ev - x0, L.a[kc], dxp);
+        bf[q] = ok ? (ev - xb) * inv : 0.0f;
+      }
+      dxp += shfl_xor_d(dxp, 16);
+      dxp += shfl_xor_d(dxp, 32);
+      double xar = x0 + dxp;
+      if (xar - maxcorr > x0) xar = x0 + maxcorr;
+      if (xar < x0 - maxcorr) xar = x0 - maxcorr;
+      f4 acc[8];
+#pragma unroll
+      for (int J = 0; J < 8; ++J) acc[J] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int p = 0; p < P128 / 8; ++p) {
+#pragma unroll
+        for (int H = 0; H < 8; H += 4) {  // four tiles at a time
+          asm volatile("" ::: "memory");
+          double2 A[4];
+#pragma unroll
+          for (int J = 0; J < 4; ++J)
+            A[J] = *reinterpret_cast<const double2*>(xr + 16 * (H + J) * LDT + 8 * p);
+#pragma unroll
+          for (int J = 0; J < 4; ++J)
+            acc[H + J] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].x, bf[2 * p], acc[H + J], 0, 0, 0);
+#pragma unroll
+          for (int J = 0; J < 4; ++J)
+            acc[H + J] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].y, bf[2 * p + 1], acc[H + J], 0, 0, 0);
+        }
+      }
+      if (rok) {
+#pragma unroll
+        for (int J = 0; J < 8; ++J) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int col = 16 * J + 4 * kk + r;
+            if (col < N) Out[(i0 + row) + (int64_t)col * ldO] = (float)(xar + sc_infl * (double)acc[J][r]);
+          }
         }
       }
     }
+    return;
+  } else {
+  for (int64_t rb = 16 * wave; rb < sz; rb += 128) {
+    if (rb != 16 * wave) load_strip(rb);
+    const int64_t row = rb + li;
+    const bool rok = row < sz;
+    double dxp = 0.0;
+#pragma unroll
+    for (int q = 0; q < P128 / 4; ++q) {
+      const bool ok = rok && (8 * (q >> 1) + 2 * kk + (q & 1)) < N;
+      b[q] = ok ? ((MODE == 1) ? (b[q] - x0) / scaling : b[q]) : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < P128 / 8; ++p) {
+      const double2 ak = *reinterpret_cast<const double2*>(L.a + 8 * p + 2 * kk);
+      dxp = fma(b[2 * p], ak.x, dxp);
+      dxp = fma(b[2 * p + 1], ak.y, dxp);
+    }
+    dxp += shfl_xor_d(dxp, 16);
+    dxp += shfl_xor_d(dxp, 32);
+    double xar = x0 + dxp;
+    if (MODE == 1) {
+      if (xar - maxcorr > x0) xar = x0 + maxcorr;
+      if (xar < x0 - maxcorr) xar = x0 - maxcorr;
+    }
+    if (MODE == 0 && rok && kk == 0) xa[i0 + row] = xar;
+    // two passes over the output tiles (4 each) keep the accumulators small
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const double* xh = xr + 64 * h * LDT;
+      d4 acc[4];
+#pragma unroll
+      for (int J = 0; J < 4; ++J) acc[J] = d4{0, 0, 0, 0};
+#pragma unroll
+      for (int p = 0; p < P128 / 8; ++p) {
+        // keeps the compiler from hoisting the reads of later pairs; the
+        // other wave on the SIMD covers the read latency
+        asm volatile("" ::: "memory");
+        double2 A[4];
+#pragma unroll
+        for (int J = 0; J < 4; ++J) A[J] = *reinterpret_cast<const double2*>(xh + 16 * J * LDT + 8 * p);
+#pragma unroll
+        for (int J = 0; J < 4; ++J) acc[J] = mfma_f64(A[J].x, b[2 * p], acc[J]);
+#pragma unroll
+        for (int J = 0; J < 4; ++J) acc[J] = mfma_f64(A[J].y, b[2 * p + 1], acc[J]);
+      }
+      if (rok) {
+#pragma unroll
+        for (int J = 0; J < 4; ++J) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int col = 64 * h + 16 * J + kk + 4 * r;
+            if (col < N) {
+              double o = acc[J][r];
+              if (MODE == 1) {
+                const double sa = (infl != 1.0) ? infl * o : o;
+                o = xar + sa * scaling;
+              }
+              Out[(i0 + row) + (int64_t)col * ldO] = o;
+            }
+          }
+        }
+      }
+    }
+  }
   }
 }
 
@@ -612,18 +788,20 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
     if (tm) tm->mark(PH_END);
     return;
   }
-  if (tm) tm->mark(PH_TMAT);
-  hipLaunchKernelGGL(k_tmat128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat, A.mat,
-                     A.vec);
+  // transform and update in one kernel (the timer books it as the update)
   if (tm) tm->mark(PH_UPDATE);
   if (A.prec == 1)
-    launch_update_f32(128, A, st);
+    hipLaunchKernelGGL((k_xform128<1, float>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
+                       A.zone_ptr, A.ymat, A.vec, A.X0, A.S32, A.ldS, A.xa, A.Out32, A.ldO,
+                       A.scaling, A.infl, A.maxcorr);
   else if (A.mode == 0)
-    hipLaunchKernelGGL(k_update128<0>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr, A.mat,
-                       A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl, A.maxcorr);
+    hipLaunchKernelGGL((k_xform128<0, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
+                       A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
+                       A.infl, A.maxcorr);
   else
-    hipLaunchKernelGGL(k_update128<1>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr, A.mat,
-                       A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl, A.maxcorr);
+    hipLaunchKernelGGL((k_xform128<1, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
+                       A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
+                       A.infl, A.maxcorr);
   if (tm) tm->mark(PH_END);
 }
 

@@ -20,11 +20,47 @@ __device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every outstanding global load and store of the wave (its fence
+// covers global memory), which exposes the full memory latency at each
+// barrier of a loop that streams results to HBM or prefetches from it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ double shfl_xor_d(double v, int mask) {
   return __shfl_xor(v, mask, 64);
 }
 
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+
+// Lane permutation of a double by DPP (two 32-bit moves, a few cycles; a
+// __shfl goes through ds_bpermute, an LDS round trip per 32-bit half).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave, returned (uniform) to every lane: quad
+// swaps and row rotations leave each 16-lane row's sum in its lanes, then the
+// four row sums are read into scalars and added.
+__device__ __forceinline__ double wave_sum_fast(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]: xor 1
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]: xor 2
+  v += dpp_d<0x124>(v);  // row_ror 4: the next quad's sum
+  v += dpp_d<0x128>(v);  // row_ror 8: the other half row
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
 
 // OAK's distance (assimilation.F90:3635-3672), p0 = observation, p1 = zone.
 // No contraction: the products and sums round exactly as OAK evaluates them,

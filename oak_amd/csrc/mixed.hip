@@ -17,7 +17,8 @@
 //                      spread (T ~ I); with |C| up to ~1e5 (precise SSH
 //                      observations) that is ~1e-3 of the spread.  Rounding
 //                      S itself perturbs those directions only at second order.
-//   k_update_f32       Ea = xa + sqrt(N-1) infl (Sf T) on v_mfma_f32_16x16x4f32
+//   k_update_f32       (N <= 64; analysis128.hip's k_xform128 for NP = 128)
+//                      Ea = xa + sqrt(N-1) infl (Sf T) on v_mfma_f32_16x16x4f32
 //                      with T rounded to FP32; xa = xbar + clamp(Sf a) in FP64
 //                      (the FP32 sum of N terms Sf_k a_k cancels strongly)
 #include "analysis.hpp"
@@ -204,7 +205,6 @@ void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st) {
     case 16: OAK_UPD32(16); break;
     case 32: OAK_UPD32(32); break;
     case 64: OAK_UPD32(64); break;
-    case 128: OAK_UPD32(128); break;
   }
 #undef OAK_UPD32
 }

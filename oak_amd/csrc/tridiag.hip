@@ -22,19 +22,16 @@
 
 namespace oakg {
 
-__device__ __forceinline__ double wave_sum_d(double s) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) s += shfl_xor_d(s, o);
-  return s;
-}
+__device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s); }
 
 // ---------------------------------------------------------------------------
 // Householder tridiagonalisation (lower, as dsytd2) with the matrix in
-// registers: thread t of 256 holds the T x T tile (rows T*(t/16).., columns
-// T*(t%16)..), T = NP/16.  Each step extracts column i through LDS, builds
-// v and tau in wave 0, forms p = tau A v from per-tile partial sums, and
-// applies A -= v w^T + w v^T (w = p - tau/2 (p.v) v) to the tiles that meet
-// the trailing block.  The reflectors go to the rows of qmat; Q is then
+// registers: thread t of 256 holds one T x T tile, T = NP/16.  Each step
+// extracts column i through LDS; every wave forms the reflector's scalars
+// from it; the tiles give their partial sums of A v and their share of
+// v^T A v (so p.v needs no second reduction); p = tau A v is summed per row;
+// and A -= v w^T + w v^T (w = p - tau/2 (p.v) v) is applied to the tiles that
+// meet the trailing block.  Three barriers per step.  The reflectors go to the rows of qmat; Q is then
 // accumulated backwards in the same tile layout and overwrites them.
 // T and Q are written in reversed order (d'_k = d_{NP-1-k}, columns of Q
 // reversed): the reduction from the top leaves the large diagonal entries at
@@ -48,7 +45,7 @@ struct TriLds {
   double col[2][NP];       // column i / Householder vector v (double-buffered)
   double pw[2][NP];        // p (then y during the Q accumulation)
   double part[16][NP + 1]; // per-tile partial sums
-  double red[2][4];        // wave partial dot products (double-buffered)
+  double red[2][4];        // per-wave shares of v^T A v (double-buffered)
   double tau[NP];
 };
 
@@ -59,7 +56,6 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
                                                  double* __restrict__ dvec, double* __restrict__ evec,
                                                  int nzc) {
   constexpr int T = NP / 16;
-  constexpr int NW = NP >= 64 ? NP / 64 : 1;  // waves that hold one entry of a length-NP vector
   __shared__ TriLds<NP> S;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
@@ -103,85 +99,83 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
           for (int ii = 0; ii < T; ++ii) col[r0 + ii] = a[ii][jj];
         }
     }
-    __syncthreads();
-    // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i]
-    if (wave == 0) {
+    lds_barrier();
+    // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i].
+    // Every wave forms the scalars from the same LDS column (identical
+    // results), so no barrier is needed before v is used.
+    double tau, scal;
+    {
       double s = 0;
       for (int r = i + 2 + lane; r < NP; r += 64) s = fma(col[r], col[r], s);
       s = wave_sum_d(s);
       const double alpha = col[i + 1];
-      const double dii = col[i];
-      double tau = 0.0, beta = alpha, scal = 0.0;
+      double beta = alpha;
+      tau = 0.0;
+      scal = 0.0;
       if (s > 0.0) {
         const double nrm = sqrt(fma(alpha, alpha, s));
         beta = alpha >= 0.0 ? -nrm : nrm;
         tau = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
       }
-      for (int r = lane; r < NP; r += 64) {
-        const double v = r <= i ? 0.0 : (r == i + 1 ? 1.0 : col[r] * scal);
-        col[r] = v;
-        V[(size_t)i * NP + r] = v;
-      }
-      if (lane == 0) {
-        dvec[(size_t)(NP - 1 - i) * nzc + zl] = dii;
-        evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
-        S.tau[i] = tau;
-      }
-    }
-    __syncthreads();
-    const double tau = S.tau[i];
-    // (3) per-tile partial sums of A v (v vanishes on rows/columns <= i)
-    const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
-    {
-      double vc[T];
-#pragma unroll
-      for (int jj = 0; jj < T; ++jj) vc[jj] = act ? col[c0 + jj] : 0.0;
-#pragma unroll
-      for (int ii = 0; ii < T; ++ii) {
-        double s = 0;
-        if (act) {
-#pragma unroll
-          for (int jj = 0; jj < T; ++jj) s = fma(a[ii][jj], vc[jj], s);
+      if (wave == 0) {
+        for (int r = lane; r < NP; r += 64)
+          V[(size_t)i * NP + r] = r <= i ? 0.0 : (r == i + 1 ? 1.0 : col[r] * scal);
+        if (lane == 0) {
+          dvec[(size_t)(NP - 1 - i) * nzc + zl] = col[i];
+          evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
+          S.tau[i] = tau;
         }
-        S.part[tc][r0 + ii] = s;
       }
     }
-    __syncthreads();
-    // (4) p = tau A v and p.v
-    {
-      double pr = 0, vr = 0;
-      if (t < NP) {
-        double s = 0;
+    // (3) per-tile partial sums of A v and the tile's share of v^T A v.
+    // The column entries are loaded unconditionally (all in one LDS batch)
+    // and v is formed from them with selects.
+    const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
+    double vR[T], vC[T];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) s += S.part[q][t];
-        pr = t > i ? tau * s : 0.0;
-        vr = col[t];
-        pw[t] = pr;
+    for (int jj = 0; jj < T; ++jj) vC[jj] = col[c0 + jj];
+#pragma unroll
+    for (int ii = 0; ii < T; ++ii) vR[ii] = col[r0 + ii];
+    auto vsel = [&](int r, double x) {
+      const double v = r == i + 1 ? 1.0 : x * scal;
+      return (act && r > i) ? v : 0.0;
+    };
+#pragma unroll
+    for (int jj = 0; jj < T; ++jj) vC[jj] = vsel(c0 + jj, vC[jj]);
+#pragma unroll
+    for (int ii = 0; ii < T; ++ii) vR[ii] = vsel(r0 + ii, vR[ii]);
+    double qf = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < T; ++ii) {
+      double s = 0;
+      if (act) {
+#pragma unroll
+        for (int jj = 0; jj < T; ++jj) s = fma(a[ii][jj], vC[jj], s);
       }
-      if (wave < NW) {
-        const double dsum = wave_sum_d(pr * vr);
-        if (lane == 0) S.red[b][wave] = dsum;
-      }
+      S.part[tc][r0 + ii] = s;
+      qf = fma(vR[ii], s, qf);
     }
-    __syncthreads();
+    qf = wave_sum_d(qf);
+    if (lane == 0) S.red[b][wave] = qf;
+    lds_barrier();
+    // (4) p = tau A v; K = -tau/2 p^T v = -tau^2/2 v^T A v
+    if (t < NP) {
+      double s = 0;
+#pragma unroll
+      for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
+      pw[t] = t > i ? tau * s : 0.0;
+    }
+    lds_barrier();
     // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
     if (act) {
-      double pv = 0;
+      const double vav = ((S.red[b][0] + S.red[b][1]) + S.red[b][2]) + S.red[b][3];
+      const double K = -0.5 * tau * tau * vav;
+      double wR[T], wC[T];
 #pragma unroll
-      for (int q = 0; q < NW; ++q) pv += S.red[b][q];
-      const double K = -0.5 * tau * pv;
-      double vR[T], wR[T], vC[T], wC[T];
+      for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
 #pragma unroll
-      for (int ii = 0; ii < T; ++ii) {
-        vR[ii] = col[r0 + ii];
-        wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
-      }
-#pragma unroll
-      for (int jj = 0; jj < T; ++jj) {
-        vC[jj] = col[c0 + jj];
-        wC[jj] = fma(K, vC[jj], pw[c0 + jj]);
-      }
+      for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[c0 + jj]);
 #pragma unroll
       for (int ii = 0; ii < T; ++ii)
 #pragma unroll
@@ -214,7 +208,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
     double* yb = S.pw[b];
     if (t < NP) vb[t] = vnext;
     if (k > 0 && t < NP) vnext = V[(size_t)(k - 1) * NP + t];  // prefetch
-    __syncthreads();
+    lds_barrier();
     const bool act = (r0 + T - 1 > k) && (c0 + T - 1 > k);
     double vR[T];
 #pragma unroll
@@ -228,14 +222,14 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       }
       S.part[tr][c0 + jj] = s;
     }
-    __syncthreads();
+    lds_barrier();
     if (t < NP) {
       double s = 0;
 #pragma unroll
       for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
       yb[t] = S.tau[k] * s;
     }
-    __syncthreads();
+    lds_barrier();
     if (act) {
 #pragma unroll
       for (int ii = 0; ii < T; ++ii)
@@ -243,7 +237,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
         for (int jj = 0; jj < T; ++jj) q[ii][jj] = fma(-vR[ii], yb[c0 + jj], q[ii][jj]);
     }
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int ii = 0; ii < T; ++ii)
 #pragma unroll
@@ -514,20 +508,20 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, int N, const int
     double s = 0;
     for (int r = hh * SEG; r < hh * SEG + SEG; ++r) s = fma(U[r * NP + j], col[ci][r], s);
     part[hh][j] = s;
-    __syncthreads();
+    lds_barrier();
     double t = 0;
     for (int q = 0; q < H; ++q) t += part[q][j];
-    __syncthreads();
+    lds_barrier();
     return t;
   };
   auto rowsum = [&](int ci) -> double {  // sum_c U[j][c] * col[ci][c]
     double s = 0;
     for (int c = hh * SEG; c < hh * SEG + SEG; ++c) s = fma(U[j * NP + c], col[ci][c], s);
     part[hh][j] = s;
-    __syncthreads();
+    lds_barrier();
     double t = 0;
     for (int q = 0; q < H; ++q) t += part[q][j];
-    __syncthreads();
+    lds_barrier();
     return t;
   };
   const double lam = dvec[(size_t)j * nzc + zl];  // eigenvalue of I + C for column j
@@ -536,39 +530,39 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, int N, const int
     col[0][j] = V[j];                  // b
     col[1][j] = j < N ? 1.0 : 0.0;     // 1_N
   }
-  __syncthreads();
+  lds_barrier();
   const double c1 = colsum(0);
   const double c2 = colsum(1);
   if (hh == 0) {
     col[2][j] = c1 / muc;
     col[3][j] = c2 * sqrt(muc);
   }
-  __syncthreads();
+  lds_barrier();
   const double a0 = rowsum(2);
   const double vr = rowsum(3);
   // refinement a += U diag(1/mu) U^T (b - M a)
   if (hh == 0) col[4][j] = a0;
-  __syncthreads();
+  lds_barrier();
   double ma = 0;
   for (int c = hh * SEG; c < hh * SEG + SEG; ++c) ma = fma(M[j * NP + c], col[4][c], ma);
   part[hh][j] = ma;
-  __syncthreads();
+  lds_barrier();
   double mat_a = 0;
   for (int q = 0; q < H; ++q) mat_a += part[q][j];
-  __syncthreads();
+  lds_barrier();
   if (hh == 0) col[5][j] = col[0][j] - mat_a;
-  __syncthreads();
+  lds_barrier();
   const double c3 = colsum(5);
   if (hh == 0) col[2][j] = c3 / muc;
-  __syncthreads();
+  lds_barrier();
   const double ar = a0 + rowsum(2);
   if (hh == 0) col[4][j] = vr * vr;
-  __syncthreads();
+  lds_barrier();
   double nv = 0;
   for (int q = 0; q < NP; ++q) nv += col[4][q];
   const double vn = vr / sqrt(nv);
   if (hh == 0) col[5][j] = 1.0 / sqrt(sqrt(muc));
-  __syncthreads();
+  lds_barrier();
   for (int e = tid; e < NP * NP; e += 256) U[e] = U[e] * col[5][e % NP];
   if (hh == 0) {
     V[NP + j] = ar;

@@ -212,6 +212,57 @@ def test_ensemble_sizes_vs_oracle(oak, N, eig, request):
         assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64, (N, infl, eig)
 
 
+@pytest.mark.parametrize("N", [100, 128])
+def test_large_zones_csr_vs_oracle(oak, N):
+    """Zones of 1, 17, 64 (no observations), 130 and 300 elements: the update
+    runs over several 16-row strips per wave and reloads strips past 128 rows."""
+    from oak_amd.rrsqrt import loc_analysis_csr
+    g = np.random.default_rng(7 + N)
+    sizes = [1, 17, 64, 130, 300]
+    zone_ptr = np.concatenate([[0], np.cumsum(sizes)])
+    n, m = int(zone_ptr[-1]), 400
+    Sf = np.asfortranarray(g.standard_normal((n, N)) / np.sqrt(N - 1))
+    HSf = np.asfortranarray(g.standard_normal((m, N)) / np.sqrt(N - 1))
+    xf = g.standard_normal(n); Hxf = g.standard_normal(m)
+    yo = Hxf + 0.3 * g.standard_normal(m)
+    sig2 = g.uniform(0.05, 0.5, m)
+    for localise_obs in (True, False):
+        # localise_obs = False needs all or none of the observations per zone
+        counts = [40, 150, 0, 300, 90] if localise_obs else [m, m, 0, m, m]
+        sp, si, sw = [0], [], []
+        for cnt in counts:
+            idx = np.sort(g.choice(m, cnt, replace=False))
+            si.extend(idx.tolist()); sw.extend(g.uniform(0.05, 1.0, cnt).tolist())
+            sp.append(sp[-1] + cnt)
+        xa, Sa, ampl = loc_analysis_csr(zone_ptr, sp, si, sw, xf, Hxf, yo, Sf, HSf, sig2,
+                                        amplitudes=True, localise_obs=localise_obs)
+        xo, So, ao = O.loc_analysis_csr(zone_ptr, sp, si, sw, xf, Hxf, yo, sig2, Sf, HSf,
+                                        localise_obs=localise_obs, want_ampl=True)
+        assert rel_err(xa - xf, xo - xf) < TOL64, (N, localise_obs)
+        assert rel_err(Sa, So) < TOL64, (N, localise_obs)
+        np.testing.assert_array_equal(Sa[17 + 1:17 + 1 + 64], Sf[17 + 1:17 + 1 + 64])
+        np.testing.assert_array_equal(xa[17 + 1:17 + 1 + 64], xf[17 + 1:17 + 1 + 64])
+
+
+@pytest.mark.parametrize("N", [100, 128])
+def test_ensemble_deep_columns_vs_oracle(oak, N):
+    """Ensemble mode with 161-element zones (40 levels): two strip rounds per wave."""
+    from oak_amd import synth
+    _, A = oak
+    P = synth.column_ocean(4, 3, 40, N, 40, 3, seed_cfg=300 + N, hlen=3, vlen=4, L=2.0,
+                           cutoff=5.0)
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    for infl, maxc in ((1.0, np.inf), (1.05, 0.1)):
+        Ea = A.ensemble_analysis(_geometry(P), H, P.Hshift, P.yo, P.rmse2, P.E, inflation=infl,
+                                 max_correction=maxc)
+        geo = dict(zone_ptr=P.zone_ptr, zpos=P.zpos, L=P.corr_len, cutoff=P.max_len, metric=0,
+                   loctype=1, ox=P.ox, oy=P.oy)
+        Eo, _, _ = O.ensemble_analysis(geo, (P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
+                                       inflation=infl, max_correction=maxc)
+        xf = P.E.mean(axis=1)
+        assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64, (N, infl)
+
+
 @pytest.mark.parametrize("metric", [1, 2])
 def test_spherical_metrics_vs_oracle(oak, metric):
     _, A = oak

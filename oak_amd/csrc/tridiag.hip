@@ -479,6 +479,164 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
 }
 
 // ---------------------------------------------------------------------------
+// U = Q Z for NP = 128 with half of each row in registers.  Thread r holds
+// columns 64..127 of row r in VGPRs and columns 0..63 in LDS (66 KB, so two
+// zones share a CU).  The chains run from their first position m-1 down to l,
+// so most rotations fall on the register half: there the chain runs as one
+// unrolled, branch-free sequence over positions 126 .. 64 with the identity
+// at the positions it does not cover (a jump into the middle of an unrolled
+// sequence is not structured control flow, and the compiler's rewriting of
+// it copies the register half around).  Position 63 couples the halves; the
+// LDS part is the loop of k_tql_apply.  The arithmetic of every rotation is
+// that of k_tql_apply.
+// ---------------------------------------------------------------------------
+struct Apply128Lds {
+  double z[8 + 128 * 65];  // columns 0..63, row stride 65, 8 doubles of room in front
+  double2 buf[2][128 + 12];
+  double2 pr[2][64];       // per wave: (c, s) of positions 64..127 for the current chain
+};
+
+__global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, const int64_t* __restrict__ sel_ptr,
+                                                      double* __restrict__ qmat,
+                                                      const double2* __restrict__ rlog,
+                                                      int64_t cap,
+                                                      const int32_t* __restrict__ status) {
+  constexpr int NP = 128, H = 64, LD = 65;
+  __shared__ Apply128Lds S;
+  double* Z = S.z + 8;
+  const int zl = blockIdx.x;
+  const int64_t z = z0 + zl;
+  if (sel_ptr[z + 1] == sel_ptr[z]) return;
+  const int stz = status[zl];
+  if (stz & 3) return;  // failed zone: solved again by the fallback
+  const int tid = threadIdx.x;
+  double* Q = qmat + (size_t)zl * NP * NP;
+  // upper half of the rows through LDS into registers, then the lower half
+  for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + H + (e & 63)];
+  __syncthreads();
+  double zr[H];
+#pragma unroll
+  for (int c = 0; c < H; ++c) zr[c] = Z[tid * LD + c];
+  __syncthreads();
+  for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + (e & 63)];
+  const int64_t len = stz >> 2;
+  const double2* lg = rlog + rlog_off(zl, cap);
+  if (tid < len) S.buf[0][tid] = lg[rlog_rec(tid)];
+  __syncthreads();
+  typedef __attribute__((address_space(3))) const double lds_d;
+  int64_t pos = 0;
+  int b = 0;
+  while (pos < len) {
+    const double2 hd = S.buf[b][0];
+    const int m = __builtin_amdgcn_readfirstlane(((int)hd.x) >> 16);
+    const int cnt = __builtin_amdgcn_readfirstlane((int)hd.y);
+    const int l = m - cnt;
+    const int64_t nxt = pos + 1 + cnt;
+    const bool pf = nxt + tid < len;
+    double2 pv = make_double2(0.0, 0.0);
+    if (pf) pv = lg[rlog_rec(nxt + tid)];
+    // record of position P: R(P) = buf[b][1 + (m - 1 - P)] = cr[2 (126 - P)], cr[2 (126 - P) + 1]
+    if (m - 1 >= H) {
+      // Positions 126 .. 64 as one unrolled sequence for every chain that
+      // reaches the register half: each wave writes the table pr[P] of its
+      // own, the chain's (c, s) for l <= P <= m-1 and the identity (1, 0)
+      // elsewhere (which leaves both columns unchanged exactly).  The LDS
+      // accesses of a wave are in order, so no barrier is needed.
+      const int lane = tid & 63;
+      double2* pr = S.pr[tid >> 6];
+      {
+        const int P = H + lane;
+        pr[lane] = (P >= l && P <= m - 1) ? S.buf[b][1 + (m - 1 - P)] : make_double2(1.0, 0.0);
+      }
+      // (c, s) read 8 positions ahead into a ring of 8 slots (slot J mod 8);
+      // the empty asm keeps each read in its place in the sequence
+      double2 qr[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) qr[(NP - 2 - t) & 7] = pr[NP - 2 - t - H];
+#pragma unroll
+      for (int J = NP - 2; J >= H; --J) {
+        const double2 q = qr[J & 7];
+        if (J - 8 >= H) qr[J & 7] = pr[J - 8 - H];
+        const double hh = zr[J + 1 - H], zj = zr[J - H];
+        zr[J + 1 - H] = fma(q.y, zj, q.x * hh);
+        zr[J - H] = fma(q.x, zj, -q.y * hh);
+        asm volatile("" ::: "memory");
+      }
+    }
+    if (l < H) {
+      // position 63 (if the chain reaches it) couples zr[0] = column 64 with
+      // column 63 in LDS; then positions 62 .. l in LDS as in k_tql_apply
+      double* p;
+      double carry;
+      int n_lo;
+      lds_d* cd;
+      if (m - 1 >= H - 1) {
+        const double2 q = S.buf[b][1 + (m - 1) - (H - 1)];
+        double* z63 = Z + tid * LD + (H - 1);
+        const double hh = zr[0], zj = *z63;
+        zr[0] = fma(q.y, zj, q.x * hh);
+        carry = fma(q.x, zj, -q.y * hh);
+        p = z63 - 1;
+        n_lo = (H - 1) - l;
+        cd = (lds_d*)(S.buf[b] + 1 + (m - 1) - (H - 2));
+      } else {
+        p = Z + tid * LD + (m - 1);
+        carry = p[1];
+        n_lo = cnt;
+        cd = (lds_d*)(S.buf[b] + 1);
+      }
+      asm volatile("" : "+v"(cd));
+      int k = 0;
+#define CS(J) make_double2(cd[2 * (J)], cd[2 * (J) + 1])
+#define OAK_ROT(Q, ZV, DST)                  \
+  DST = fma(Q.y, ZV, Q.x * carry);           \
+  carry = fma(Q.x, ZV, -Q.y * carry);
+      if (n_lo >= 8) {
+        double a0 = p[0], a1 = p[-1], a2 = p[-2], a3 = p[-3];
+        double2 qa0 = CS(0), qa1 = CS(1), qa2 = CS(2), qa3 = CS(3);
+        for (; k + 8 <= n_lo; k += 8) {
+          const double b0 = p[-4], b1 = p[-5], b2 = p[-6], b3 = p[-7];
+          const double2 qb0 = CS(4), qb1 = CS(5), qb2 = CS(6), qb3 = CS(7);
+          OAK_ROT(qa0, a0, p[1])
+          OAK_ROT(qa1, a1, p[0])
+          OAK_ROT(qa2, a2, p[-1])
+          OAK_ROT(qa3, a3, p[-2])
+          a0 = p[-8]; a1 = p[-9]; a2 = p[-10]; a3 = p[-11];
+          qa0 = CS(8); qa1 = CS(9); qa2 = CS(10); qa3 = CS(11);
+          OAK_ROT(qb0, b0, p[-3])
+          OAK_ROT(qb1, b1, p[-4])
+          OAK_ROT(qb2, b2, p[-5])
+          OAK_ROT(qb3, b3, p[-6])
+          p -= 8;
+          cd += 16;
+        }
+      }
+      for (; k < n_lo; ++k) {
+        const double zv = p[0];
+        const double2 q = CS(0);
+        OAK_ROT(q, zv, p[1])
+        p -= 1;
+        cd += 2;
+      }
+#undef OAK_ROT
+#undef CS
+      p[1] = carry;  // column l
+    }
+    if (pf) S.buf[b ^ 1][tid] = pv;
+    __syncthreads();
+    pos = nxt;
+    b ^= 1;
+  }
+  // lower half out, then the upper half through LDS
+  for (int e = tid; e < NP * H; e += NP) Q[(e >> 6) * NP + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zr[c];
+  __syncthreads();
+  for (int e = tid; e < NP * H; e += NP) Q[(e >> 6) * NP + H + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
+}
+
+// ---------------------------------------------------------------------------
 // Epilogue from eigenvectors U (rows in global memory) and eigenvalues d.
 // ---------------------------------------------------------------------------
 template <int NP>
@@ -580,8 +738,12 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
                      A.dvec, A.evec, A.nzc);
   hipLaunchKernelGGL(k_tql<NP>, dim3((A.nzc + 63) / 64), dim3(64), 0, st, A.nzc, A.z0, A.sel_ptr,
                      A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
-  hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0, A.sel_ptr, A.ymat,
-                     (const double2*)A.rlog, A.rlog_cap, A.tql_status);
+  if (NP == 128)
+    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(128), 0, st, A.z0, A.sel_ptr, A.ymat,
+                       (const double2*)A.rlog, A.rlog_cap, A.tql_status);
+  else
+    hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0,
+                       A.sel_ptr, A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status);
   hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
                      A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
 }

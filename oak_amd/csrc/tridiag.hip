@@ -17,6 +17,8 @@
 //                  against M), v ~ U diag(mu^1/2) U^T 1, Y = U diag(mu^-1/4),
 //                  with mu = max(lambda, 1) (rrsqrt.F90:137-142, 176-177);
 //                  mu itself is kept beside a and v for the likelihood.
+#include <type_traits>
+
 #include "analysis.hpp"
 #include "common.hpp"
 
@@ -482,11 +484,11 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
 // U = Q Z for NP = 128 with half of each row in registers.  Thread r holds
 // columns 64..127 of row r in VGPRs and columns 0..63 in LDS (66 KB, so two
 // zones share a CU).  The chains run from their first position m-1 down to l,
-// so most rotations fall on the register half: there the chain runs as one
-// unrolled, branch-free sequence over positions 126 .. 64 with the identity
-// at the positions it does not cover (a jump into the middle of an unrolled
-// sequence is not structured control flow, and the compiler's rewriting of
-// it copies the register half around).  Position 63 couples the halves; the
+// so most rotations fall on the register half: there the chain runs as
+// unrolled, branch-free segments of 16 positions, each segment it meets with
+// the identity at the positions it does not cover (a jump into the middle of
+// an unrolled sequence is not structured control flow, and the compiler's
+// rewriting of it copies the register half around).  Position 63 couples the halves; the
 // LDS part is the loop of k_tql_apply.  The arithmetic of every rotation is
 // that of k_tql_apply.
 // ---------------------------------------------------------------------------
@@ -548,20 +550,30 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, const int64_t*
         const int P = H + lane;
         pr[lane] = (P >= l && P <= m - 1) ? S.buf[b][1 + (m - 1 - P)] : make_double2(1.0, 0.0);
       }
-      // (c, s) read 8 positions ahead into a ring of 8 slots (slot J mod 8);
-      // the empty asm keeps each read in its place in the sequence
-      double2 qr[8];
+      // four segments of 16 positions (126..111, 110..95, 94..79, 78..64),
+      // each run only if the chain meets it; a segment's (c, s) are read at
+      // its start
+      auto segment = [&](auto top) {
+        constexpr int T = decltype(top)::value;
+        if (m - 1 < T - 15 || l > T) return;
+        double2 qv[16];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) qr[(NP - 2 - t) & 7] = pr[NP - 2 - t - H];
+        for (int t = 0; t < 16; ++t)
+          if (T - t >= H) qv[t] = pr[T - t - H];
 #pragma unroll
-      for (int J = NP - 2; J >= H; --J) {
-        const double2 q = qr[J & 7];
-        if (J - 8 >= H) qr[J & 7] = pr[J - 8 - H];
-        const double hh = zr[J + 1 - H], zj = zr[J - H];
-        zr[J + 1 - H] = fma(q.y, zj, q.x * hh);
-        zr[J - H] = fma(q.x, zj, -q.y * hh);
-        asm volatile("" ::: "memory");
-      }
+        for (int t = 0; t < 16; ++t) {
+            const int J = T - t;
+          if (J >= H) {
+            const double hh = zr[J + 1 - H], zj = zr[J - H];
+            zr[J + 1 - H] = fma(qv[t].y, zj, qv[t].x * hh);
+            zr[J - H] = fma(qv[t].x, zj, -qv[t].y * hh);
+          }
+        }
+      };
+      segment(std::integral_constant<int, 126>{});
+      segment(std::integral_constant<int, 110>{});
+      segment(std::integral_constant<int, 94>{});
+      segment(std::integral_constant<int, 78>{});
     }
     if (l < H) {
       // position 63 (if the chain reaches it) couples zr[0] = column 64 with

@@ -6,7 +6,7 @@
 //   k_tridiag      one workgroup per zone, matrix in LDS: M = Q T Q^T with
 //                  Householder reflectors (lower), then Q formed explicitly
 //                  by backward accumulation.  Writes Q and (d, e).
-//   k_tql          one lane per zone: implicit QL with Wilkinson shifts on
+//   k_tql          one lane per zone (16 zones per wave): implicit QL with Wilkinson shifts on
 //                  (d, e) (Golub & Van Loan, 4th ed., section 8.3; Bowdler,
 //                  Martin, Reinsch & Wilkinson 1968); the sequential
 //                  recurrences of many zones run side by side.  Every plane
@@ -273,16 +273,20 @@ __device__ __forceinline__ double rsqrt_acc(double h) {
 // end m of the unreduced block is carried from one sweep to the next: the
 // negligibility test of e_j is made as soon as e_j, d_j and d_{j+1} are final
 // in the sweep, so a full rescan is only needed after a 1 x 1 deflation.
-template <int NP>
+template <int NP, int ZW>
 __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* __restrict__ sel_ptr,
                                             double* __restrict__ dvec, double* __restrict__ evec,
                                             double2* __restrict__ rlog, int64_t cap,
                                             int32_t* __restrict__ status,
                                             unsigned* __restrict__ err) {
-  __shared__ double d[NP][64];
-  __shared__ double e[NP][64];
+  // ZW zones per wave (lanes ZW..63 idle): the sweeps of different zones
+  // differ in length and number, and a wave runs the longest of its lanes,
+  // so fewer zones per wave waste less and spread the work over more SIMDs
+  __shared__ double d[NP][ZW];
+  __shared__ double e[NP][ZW];
   const int lane = threadIdx.x;
-  const int zl = blockIdx.x * 64 + lane;
+  if (lane >= ZW) return;
+  const int zl = blockIdx.x * ZW + lane;
   const bool valid = zl < nzc && sel_ptr[z0 + zl + 1] > sel_ptr[z0 + zl];
   if (valid) {
     for (int k = 0; k < NP; ++k) {
@@ -748,8 +752,9 @@ template <int NP>
 static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   hipLaunchKernelGGL(k_tridiag<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.mat, A.ymat,
                      A.dvec, A.evec, A.nzc);
-  hipLaunchKernelGGL(k_tql<NP>, dim3((A.nzc + 63) / 64), dim3(64), 0, st, A.nzc, A.z0, A.sel_ptr,
-                     A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
+  constexpr int ZW = 16;
+  hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0,
+                     A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
   if (NP == 128)
     hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(128), 0, st, A.z0, A.sel_ptr, A.ymat,
                        (const double2*)A.rlog, A.rlog_cap, A.tql_status);

@@ -500,22 +500,31 @@ struct Apply128Lds {
   double z[8 + 128 * 65];  // columns 0..63, row stride 65, 8 doubles of room in front
   double2 buf[2][128 + 12];
   double2 pr[2][64];       // per wave: (c, s) of positions 64..127 for the current chain
+  double ev[5][128];       // epilogue vectors
 };
 
-__global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+                                                      const double* __restrict__ mat,
                                                       double* __restrict__ qmat,
                                                       const double2* __restrict__ rlog,
                                                       int64_t cap,
-                                                      const int32_t* __restrict__ status) {
+                                                      const int32_t* __restrict__ status,
+                                                      const double* __restrict__ dvec, int nzc,
+                                                      double* __restrict__ vec,
+                                                      double* __restrict__ ampl,
+                                                      unsigned* __restrict__ err) {
   constexpr int NP = 128, H = 64, LD = 65;
   __shared__ Apply128Lds S;
   double* Z = S.z + 8;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
-  if (sel_ptr[z + 1] == sel_ptr[z]) return;
+  const int tid = threadIdx.x;
+  if (sel_ptr[z + 1] == sel_ptr[z]) {  // no observations: a = 0
+    vec[(size_t)zl * 4 * NP + NP + tid] = 0.0;
+    return;
+  }
   const int stz = status[zl];
   if (stz & 3) return;  // failed zone: solved again by the fallback
-  const int tid = threadIdx.x;
   double* Q = qmat + (size_t)zl * NP * NP;
   // upper half of the rows through LDS into registers, then the lower half
   for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + H + (e & 63)];
@@ -643,11 +652,140 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, const int64_t*
     pos = nxt;
     b ^= 1;
   }
-  // lower half out, then the upper half through LDS
+  // ---- epilogue: the quantities of k_eigvec_epi from the rows in hand ----
+  // Thread r holds the upper half of row r of U in registers; the lower half
+  // is in LDS and, while the upper half is staged there, in Q (the output
+  // location, L2-resident).  Column sums U^T w are taken from the half staged
+  // in LDS (column cc, rows 64 hf .. 64 hf + 63 per thread); row products U x
+  // from the registers and the lower half.
+  double* mu = S.ev[0];  // mu_j = max(lambda_j, 1) of column j
+  double* bv = S.ev[1];  // b
+  double* xv = S.ev[2];  // U^T b / mu, then U^T (b - M a0) / mu
+  double* yv = S.ev[3];  // U^T 1_N sqrt(mu), then mu^-1/4
+  double* av = S.ev[4];  // a0, then b - M a0, then vr^2
+  double* part = reinterpret_cast<double*>(S.buf);  // [4][64] partial column sums
+  double* V = vec + (size_t)zl * 4 * NP;
+  mu[tid] = fmax(dvec[(size_t)tid * nzc + zl], 1.0);
+  bv[tid] = V[tid];
+  const int cc = tid & 63, hf = tid >> 6;
+  __syncthreads();
+  // U^T b and U^T 1_N over the staged half (column offset co)
+  auto colsum2 = [&](int co) {
+    double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+#pragma unroll 4
+    for (int r0 = 64 * hf; r0 < 64 * hf + 64; r0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + u;
+        const double zz = Z[r * LD + cc];
+        s1[u] = fma(zz, bv[r], s1[u]);
+        s2[u] += r < N ? zz : 0.0;
+      }
+    }
+    part[hf * 64 + cc] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    part[(2 + hf) * 64 + cc] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+    __syncthreads();
+    if (tid < 64) {
+      const double c1 = part[tid] + part[64 + tid];
+      const double c2 = part[128 + tid] + part[192 + tid];
+      xv[co + tid] = c1 / mu[co + tid];
+      yv[co + tid] = c2 * sqrt(mu[co + tid]);
+    }
+    __syncthreads();
+  };
+  // U^T w (w = av) over the staged half, divided by mu, into xv
+  auto colsum1 = [&](int co) {
+    double s1[4] = {0, 0, 0, 0};
+#pragma unroll 4
+    for (int r0 = 64 * hf; r0 < 64 * hf + 64; r0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s1[u] = fma(Z[(r0 + u) * LD + cc], av[r0 + u], s1[u]);
+    }
+    part[hf * 64 + cc] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    __syncthreads();
+    if (tid < 64) xv[co + tid] = (part[tid] + part[64 + tid]) / mu[co + tid];
+    __syncthreads();
+  };
+  // (U x)_r and (U y)_r in one pass: the lower half of row r from lo (LDS row
+  // or global row), the upper half from the registers; 16 entries at a time
+  auto rowdot2 = [&](const double* lo, const double* x, const double* y, double& tx, double& ty) {
+    tx = 0;
+    ty = 0;
+#pragma unroll
+    for (int c0 = 0; c0 < NP; c0 += 16) {
+      asm volatile("" ::: "memory");
+      double2 uu[8], xx[8], yy[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (c0 < H) uu[u] = reinterpret_cast<const double2*>(lo + c0)[u];
+        else uu[u] = make_double2(zr[c0 - H + 2 * u], zr[c0 - H + 2 * u + 1]);
+        xx[u] = reinterpret_cast<const double2*>(x + c0)[u];
+        if (y) yy[u] = reinterpret_cast<const double2*>(y + c0)[u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        tx = fma(uu[u].x, xx[u].x, tx);
+        tx = fma(uu[u].y, xx[u].y, tx);
+        if (y) {
+          ty = fma(uu[u].x, yy[u].x, ty);
+          ty = fma(uu[u].y, yy[u].y, ty);
+        }
+      }
+    }
+  };
+  colsum2(0);  // the lower half is in LDS
   for (int e = tid; e < NP * H; e += NP) Q[(e >> 6) * NP + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < H; ++c) Z[tid * LD + c] = zr[c];
+  __threadfence_block();  // the lower half in Q is read back by its own row's thread
+  __syncthreads();
+  colsum2(H);  // upper half
+  double a0, vr;
+  rowdot2(Q + (size_t)tid * NP, xv, yv, a0, vr);
+  av[tid] = a0;
+  __syncthreads();
+  // refinement a = a0 + U diag(1/mu) U^T (b - M a0); M symmetric: column tid
+  const double* M = mat + (size_t)zl * NP * NP;
+  double mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 2
+  for (int c0 = 0; c0 < NP; c0 += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) mq[u] = fma(M[(size_t)(c0 + u) * NP + tid], av[c0 + u], mq[u]);
+  }
+  const double ma = ((mq[0] + mq[1]) + (mq[2] + mq[3])) + ((mq[4] + mq[5]) + (mq[6] + mq[7]));
+  __syncthreads();
+  av[tid] = bv[tid] - ma;
+  __syncthreads();
+  colsum1(H);  // upper half still staged
+  for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + (e & 63)];
+  __syncthreads();
+  colsum1(0);
+  double ux, unused;
+  rowdot2(Z + tid * LD, xv, nullptr, ux, unused);
+  const double ar = a0 + ux;
+  av[tid] = vr * vr;
+  yv[tid] = 1.0 / sqrt(sqrt(mu[tid]));
+  __syncthreads();
+  double nq[4] = {0, 0, 0, 0};
+#pragma unroll 8
+  for (int q = 0; q < NP; q += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nq[u] += av[q + u];
+  }
+  const double nv = (nq[0] + nq[1]) + (nq[2] + nq[3]);
+  const double vn = vr / sqrt(nv);
+  V[NP + tid] = ar;
+  V[2 * NP + tid] = vn;
+  V[3 * NP + tid] = mu[tid];
+  if (ampl && tid < N) ampl[(size_t)z * N + tid] = ar;
+  if (ar != ar) atomicOr(err, OAKG_DERR_NAN);
+  // Y = U diag(mu^-1/4): the lower half from LDS, then the upper half through LDS
+  for (int e = tid; e < NP * H; e += NP)
+    Q[(e >> 6) * NP + (e & 63)] = Z[(e >> 6) * LD + (e & 63)] * yv[e & 63];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zr[c] * yv[H + c];
   __syncthreads();
   for (int e = tid; e < NP * H; e += NP) Q[(e >> 6) * NP + H + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
 }
@@ -755,14 +893,16 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   constexpr int ZW = 16;
   hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0,
                      A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
-  if (NP == 128)
-    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(128), 0, st, A.z0, A.sel_ptr, A.ymat,
-                       (const double2*)A.rlog, A.rlog_cap, A.tql_status);
-  else
+  if (NP == 128) {  // the epilogue runs at the end of the apply kernel
+    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(128), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
+                       A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,
+                       A.vec, A.ampl, A.err);
+  } else {
     hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0,
                        A.sel_ptr, A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status);
-  hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
-                     A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
+    hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
+  }
 }
 
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st) {

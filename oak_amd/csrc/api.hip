@@ -34,6 +34,9 @@ struct oakg_ctx {
   const int64_t* last_sel_ptr = nullptr;
   int64_t* d_stats = nullptr;  // device: error word + reduced statistics
   bool counts_ready = true;
+  // second stream for the chunk pipeline (default priority) and its events
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace oakg {
@@ -300,6 +303,12 @@ void materialize_counts(oakg_ctx* c) {
 }
 
 // Runs the chunked zone loop (gram, eigensolve, tmat, update) for all zones.
+// Without phase timing, consecutive chunks alternate between the caller's
+// stream and a second stream, each with its own half of the chunk
+// workspace, so that the tail of one chunk's kernels (the QL sweeps of its
+// slowest zones, above all) overlaps the next chunk's work.  The streams are
+// joined by events before the call continues.  With phase timing the chunks
+// run in order on the caller's stream (the timer's events are on that stream).
 void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_all) {
   // QL rotation log: records of 16 B per zone, a multiple of 4 (the log is
   // interleaved in groups of 4 records over the 64 zones of a QL wave)
@@ -309,39 +318,60 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
   const size_t budget = (size_t)8 << 30;
-  int64_t zc = std::min<int64_t>(nz, std::max<int64_t>(1, (int64_t)(budget / per_zone)));
+  const int nslot = (!c->timing && nz >= 4096) ? 2 : 1;
+  int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
   zc = std::min<int64_t>(zc, 1 << 20);
-  double* mat = (double*)c->ws.get(Workspace::MAT, sizeof(double) * (size_t)NP * NP * zc);
-  double* ymat = (double*)c->ws.get(Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zc);
-  double* vec = (double*)c->ws.get(Workspace::VEC, sizeof(double) * (size_t)4 * NP * zc);
+  if (nslot == 2) zc = std::min<int64_t>(zc, (nz + 1) / 2);  // at least two chunks
+  zc = std::min<int64_t>(zc, nz);
+  zc = (zc + 63) & ~(int64_t)63;  // slots start on 64-zone groups of the rotation log
+  const int64_t zs = zc * nslot;
+  double* mat = (double*)c->ws.get(Workspace::MAT, sizeof(double) * (size_t)NP * NP * zs);
+  double* ymat = (double*)c->ws.get(Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zs);
+  double* vec = (double*)c->ws.get(Workspace::VEC, sizeof(double) * (size_t)4 * NP * zs);
   if (!mat || !ymat || !vec) check(c, hipErrorOutOfMemory, "workspace");
   double *dvec = nullptr, *evec = nullptr;
-  void* rlog = nullptr;
+  char* rlog = nullptr;
   int32_t* tst = nullptr;
   if (ql) {
-    dvec = (double*)c->ws.get(Workspace::DVEC, sizeof(double) * (size_t)NP * zc);
-    evec = (double*)c->ws.get(Workspace::EVEC, sizeof(double) * (size_t)NP * zc);
-    rlog = c->ws.get(Workspace::RLOG, (size_t)16 * cap * ((zc + 63) & ~(int64_t)63));
-    tst = (int32_t*)c->ws.get(Workspace::TQL_STATUS, sizeof(int32_t) * zc);
+    dvec = (double*)c->ws.get(Workspace::DVEC, sizeof(double) * (size_t)NP * zs);
+    evec = (double*)c->ws.get(Workspace::EVEC, sizeof(double) * (size_t)NP * zs);
+    rlog = (char*)c->ws.get(Workspace::RLOG, (size_t)16 * cap * zs);
+    tst = (int32_t*)c->ws.get(Workspace::TQL_STATUS, sizeof(int32_t) * zs);
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
     fill_i32(sweeps_all, nz, 0, c->stream);
   }
-  for (int64_t z0 = 0; z0 < nz; z0 += zc) {
+  hipStream_t S[2] = {c->stream, c->stream};
+  if (nslot == 2) {
+    if (!c->aux) check(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "stream");
+    if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
+    if (!c->ev_join) check(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event");
+    S[1] = c->aux;
+    check(c, hipEventRecord(c->ev_fork, S[0]), "event");
+    check(c, hipStreamWaitEvent(S[1], c->ev_fork, 0), "stream wait");
+  }
+  int64_t k = 0;
+  for (int64_t z0 = 0; z0 < nz; z0 += zc, ++k) {
+    const int sl = (int)(k % nslot);
+    const size_t zo = (size_t)sl * zc;  // first zone of the slot's workspace
     ChunkArgs A = base;
     A.z0 = z0;
     A.nzc = (int)std::min<int64_t>(zc, nz - z0);
-    A.mat = mat;
-    A.ymat = ymat;
-    A.vec = vec;
+    A.mat = mat + zo * NP * NP;
+    A.ymat = ymat + zo * NP * NP;
+    A.vec = vec + zo * 4 * NP;
     A.sweeps = sweeps_all + z0;
     A.eig_method = ql ? 1 : 0;
-    A.dvec = dvec;
-    A.evec = evec;
-    A.rlog = rlog;
+    A.dvec = ql ? dvec + zo * NP : nullptr;
+    A.evec = ql ? evec + zo * NP : nullptr;
+    A.rlog = ql ? (void*)(rlog + (size_t)16 * cap * zo) : nullptr;
     A.rlog_cap = cap;
-    A.tql_status = tst;
-    launch_chunk(NP, A, c->stream, c->timing ? &c->timer : nullptr);
+    A.tql_status = ql ? tst + zo : nullptr;
+    launch_chunk(NP, A, S[sl], c->timing ? &c->timer : nullptr);
     check(c, hipGetLastError(), "chunk launch");
+  }
+  if (nslot == 2) {
+    check(c, hipEventRecord(c->ev_join, S[1]), "event");
+    check(c, hipStreamWaitEvent(S[0], c->ev_join, 0), "stream wait");
   }
 }
 
@@ -782,6 +812,12 @@ int oakg_destroy(oakg_ctx* c) {
   if (!c) return OAKG_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->d_err) (void)hipFree(c->d_err);
   if (c->d_stats) (void)hipFree(c->d_stats);
   c->ws.release();

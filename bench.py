@@ -144,7 +144,7 @@ def run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea, label):
     separate passes with phase timing on, where the chunks run in order."""
     import torch
     from oak_amd.assimilation import ensemble_analysis
-    ctx.set_timing(False)
+    ctx.set_timing(args.serial)
     for i in range(args.warmup):
         ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
         torch.cuda.synchronize()
@@ -304,6 +304,9 @@ def main():
     ap.add_argument("--eig", default="ql", choices=["ql", "jacobi"],
                     help="eigensolver: tridiagonal QL (default) or one-sided Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="run every step with phase timing (zone chunks in order on one stream), "
+                         "as the rocprofv3 kernel summaries are taken")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--calib-only", action="store_true",

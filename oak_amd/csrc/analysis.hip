@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_prep_loc(int64_t m, int N, const double
   for (int e = threadIdx.x; e < TR * NP; e += 256) {
     int li = e / NP, k = e % NP;
     int64_t l = l0 + li;
-    if (l < m) HS[l * NP + k] = tile[li][k] * scale;
+    if (l < m) HS[l * NP + hs_col<NP>(k)] = tile[li][k] * scale;
   }
 }
 
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_obs_anom(int64_t m, int N, const double
   for (int e = threadIdx.x; e < TR * NP; e += 256) {
     int li = e / NP, k = e % NP;
     int64_t l = l0 + li;
-    if (l < m) HS[l * NP + k] = (k < N) ? (tile[li][k] - ybar[li]) / scaling : 0.0;
+    if (l < m) HS[l * NP + hs_col<NP>(k)] = (k < N) ? (tile[li][k] - ybar[li]) / scaling : 0.0;
   }
 }
 

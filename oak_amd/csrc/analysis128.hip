@@ -60,10 +60,24 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
         rd = r * innov[l];
       }
     }
+    // columns 16 J + jl, J = 0..7, stored contiguously (hs_col<128>)
     double bv[8];
-    const TS* row = HS + (size_t)l * P128 + jl;
+    const TS* row = HS + (size_t)l * P128 + 8 * jl;
+    if constexpr (sizeof(TS) == 4) {
+      const float4 u0 = reinterpret_cast<const float4*>(row)[0];
+      const float4 u1 = reinterpret_cast<const float4*>(row)[1];
+      bv[0] = u0.x; bv[1] = u0.y; bv[2] = u0.z; bv[3] = u0.w;
+      bv[4] = u1.x; bv[5] = u1.y; bv[6] = u1.z; bv[7] = u1.w;
+    } else {
 #pragma unroll
-    for (int J = 0; J < 8; ++J) bv[J] = ok ? (double)row[16 * J] : 0.0;
+      for (int h = 0; h < 4; ++h) {
+        const double2 u = reinterpret_cast<const double2*>(row)[h];
+        bv[2 * h] = u.x;
+        bv[2 * h + 1] = u.y;
+      }
+    }
+#pragma unroll
+    for (int J = 0; J < 8; ++J) bv[J] = ok ? bv[J] : 0.0;
     const double aA = r * bv[IA], aB = r * bv[IB];
 #pragma unroll
     for (int J = 0; J <= IA; ++J) accA[J] = mfma_f64(aA, bv[J], accA[J]);

@@ -30,6 +30,15 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Column j of a row of the internal observation-anomaly matrix HS (m x NP)
+// is stored at hs_col<NP>(j).  For NP = 128 the 8 columns 16 J + c (J = 0..7)
+// that lane c of the Gram MFMA needs are contiguous (8 c + J), so a lane
+// reads them with 16-byte loads; for NP <= 64 the order is the natural one.
+template <int NP>
+__host__ __device__ constexpr int hs_col(int j) {
+  return NP == 128 ? 8 * (j & 15) + (j >> 4) : j;
+}
+
 __device__ __forceinline__ double shfl_xor_d(double v, int mask) {
   return __shfl_xor(v, mask, 64);
 }

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_obs_anom_f32(int64_t m, int N, const do
   for (int e = threadIdx.x; e < TR * NP; e += 256) {
     int li = e / NP, k = e % NP;
     int64_t l = l0 + li;
-    if (l < m) HS[l * NP + k] = (k < N) ? (float)((tile[li][k] - ybar[li]) * inv) : 0.0f;
+    if (l < m) HS[l * NP + hs_col<NP>(k)] = (k < N) ? (float)((tile[li][k] - ybar[li]) * inv) : 0.0f;
   }
 }
 

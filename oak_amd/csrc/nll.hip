@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_nll_adjoint(int64_t z0, int N,
 #pragma unroll
     for (int c = 0; c < KPL; ++c) {
       const int j = lane + 64 * c;
-      const double s = j < NP ? srow[j] : 0.0;
+      const double s = j < NP ? srow[hs_col<NP>(j)] : 0.0;
       if (j < NP) sw[wave][j] = s;
       t = fma(s, ak[c], t);
     }

@@ -35,9 +35,13 @@ struct oakg_ctx {
   int64_t* d_stats = nullptr;  // device: error word + reduced statistics
   bool counts_ready = true;
   // second stream for the chunk pipeline (default priority) and its events
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
 };
+
+#ifndef OAKG_PIPE_SLOTS
+#define OAKG_PIPE_SLOTS 3  // zone chunks in flight on as many streams
+#endif
 
 namespace oakg {
 
@@ -303,10 +307,10 @@ void materialize_counts(oakg_ctx* c) {
 }
 
 // Runs the chunked zone loop (gram, eigensolve, tmat, update) for all zones.
-// Without phase timing, consecutive chunks alternate between the caller's
-// stream and a second stream, each with its own half of the chunk
-// workspace, so that the tail of one chunk's kernels (the QL sweeps of its
-// slowest zones, above all) overlaps the next chunk's work.  The streams are
+// Without phase timing, consecutive chunks rotate over the caller's stream
+// and OAKG_PIPE_SLOTS - 1 internal streams, each with its own part of the
+// chunk workspace, so that the tail of one chunk's kernels (the QL sweeps of
+// its slowest zones, above all) overlaps the next chunks' work.  The streams are
 // joined by events before the call continues.  With phase timing the chunks
 // run in order on the caller's stream (the timer's events are on that stream).
 void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_all) {
@@ -318,10 +322,10 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
   const size_t budget = (size_t)8 << 30;
-  const int nslot = (!c->timing && nz >= 4096) ? 2 : 1;
+  const int nslot = (!c->timing && nz >= 4096) ? OAKG_PIPE_SLOTS : 1;
   int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
   zc = std::min<int64_t>(zc, 1 << 20);
-  if (nslot == 2) zc = std::min<int64_t>(zc, (nz + 1) / 2);  // at least two chunks
+  if (nslot > 1) zc = std::min<int64_t>(zc, (nz + nslot - 1) / nslot);  // a chunk per slot
   zc = std::min<int64_t>(zc, nz);
   zc = (zc + 63) & ~(int64_t)63;  // slots start on 64-zone groups of the rotation log
   const int64_t zs = zc * nslot;
@@ -340,14 +344,18 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
     fill_i32(sweeps_all, nz, 0, c->stream);
   }
-  hipStream_t S[2] = {c->stream, c->stream};
-  if (nslot == 2) {
-    if (!c->aux) check(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "stream");
+  hipStream_t S[3] = {c->stream, c->stream, c->stream};
+  if (nslot > 1) {
     if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
-    if (!c->ev_join) check(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event");
-    S[1] = c->aux;
     check(c, hipEventRecord(c->ev_fork, S[0]), "event");
-    check(c, hipStreamWaitEvent(S[1], c->ev_fork, 0), "stream wait");
+    for (int i = 1; i < nslot; ++i) {
+      if (!c->aux[i - 1])
+        check(c, hipStreamCreateWithFlags(&c->aux[i - 1], hipStreamNonBlocking), "stream");
+      if (!c->ev_join[i - 1])
+        check(c, hipEventCreateWithFlags(&c->ev_join[i - 1], hipEventDisableTiming), "event");
+      S[i] = c->aux[i - 1];
+      check(c, hipStreamWaitEvent(S[i], c->ev_fork, 0), "stream wait");
+    }
   }
   int64_t k = 0;
   for (int64_t z0 = 0; z0 < nz; z0 += zc, ++k) {
@@ -369,9 +377,9 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     launch_chunk(NP, A, S[sl], c->timing ? &c->timer : nullptr);
     check(c, hipGetLastError(), "chunk launch");
   }
-  if (nslot == 2) {
-    check(c, hipEventRecord(c->ev_join, S[1]), "event");
-    check(c, hipStreamWaitEvent(S[0], c->ev_join, 0), "stream wait");
+  for (int i = 1; i < nslot; ++i) {
+    check(c, hipEventRecord(c->ev_join[i - 1], S[i]), "event");
+    check(c, hipStreamWaitEvent(S[0], c->ev_join[i - 1], 0), "stream wait");
   }
 }
 
@@ -812,12 +820,14 @@ int oakg_destroy(oakg_ctx* c) {
   if (!c) return OAKG_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  if (c->aux) {
-    (void)hipStreamSynchronize(c->aux);
-    (void)hipStreamDestroy(c->aux);
+  for (int i = 0; i < 2; ++i) {
+    if (c->aux[i]) {
+      (void)hipStreamSynchronize(c->aux[i]);
+      (void)hipStreamDestroy(c->aux[i]);
+    }
+    if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->d_err) (void)hipFree(c->d_err);
   if (c->d_stats) (void)hipFree(c->d_stats);
   c->ws.release();

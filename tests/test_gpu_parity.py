@@ -263,6 +263,28 @@ def test_ensemble_deep_columns_vs_oracle(oak, N):
         assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64, (N, infl)
 
 
+@pytest.mark.parametrize("N", [16, 128])
+def test_chunk_pipeline_matches_in_order(oak, N):
+    """Without phase timing the zone chunks run on several streams; with it
+    they run in order on one stream.  Every zone is computed alone, so the
+    two schedules (and their different chunk sizes) give the same bits."""
+    from oak_amd import synth
+    from oak_amd._lib import Context
+    _, A = oak
+    P = synth.column_ocean(64, 64, 1, N, 3000, 200, seed_cfg=500 + N, hlen=4, vlen=1, L=3.0,
+                           cutoff=9.0)
+    assert P.nzones >= 4096  # the pipeline runs from 4096 zones on
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    out = []
+    for timing in (False, True):
+        c = Context(0)
+        c.set_timing(timing)
+        out.append(A.ensemble_analysis(_geometry(P), H, P.Hshift, P.yo, P.rmse2, P.E, ctx=c))
+        c.close()
+    np.testing.assert_array_equal(out[0], out[1])
+    assert not np.array_equal(out[0], P.E)
+
+
 @pytest.mark.parametrize("metric", [1, 2])
 def test_spherical_metrics_vs_oracle(oak, metric):
     _, A = oak

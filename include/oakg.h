@@ -117,9 +117,14 @@ int oakg_version(void);
 int oakg_create(oakg_ctx** ctx, int device, int precision);
 int oakg_destroy(oakg_ctx* ctx);
 const char* oakg_last_error(const oakg_ctx* ctx);
-/* Launch on this HIP stream (hipStream_t) instead of the default stream. */
+/* Launch on this HIP stream (hipStream_t) instead of the default stream.
+   With 4096 zones or more (and phase timing off) the zone chunks also run on
+   two internal non-blocking streams of the context (default priority),
+   ordered after the work already on `stream` and joined back into it by
+   events before the call returns; the call synchronises `stream`. */
 int oakg_set_stream(oakg_ctx* ctx, void* stream);
-/* 1 = record per-phase device times into oakg_stats (adds events). */
+/* 1 = record per-phase device times into oakg_stats (adds events; the zone
+   chunks then run in order on the context's stream). */
 int oakg_set_timing(oakg_ctx* ctx, int on);
 /* Eigensolver of the N x N ensemble-space matrix (OAK calls LAPACK dsyev,
    matoper_inc.F90:986-1005):

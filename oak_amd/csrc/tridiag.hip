@@ -3,20 +3,25 @@
 // which OAK's symeig calls, matoper_inc.F90:986-1005), restructured for a
 // batch of zones:
 //
-//   k_tridiag      one workgroup per zone, matrix in LDS: M = Q T Q^T with
-//                  Householder reflectors (lower), then Q formed explicitly
-//                  by backward accumulation.  Writes Q and (d, e).
-//   k_tql          one lane per zone (16 zones per wave): implicit QL with Wilkinson shifts on
-//                  (d, e) (Golub & Van Loan, 4th ed., section 8.3; Bowdler,
-//                  Martin, Reinsch & Wilkinson 1968); the sequential
-//                  recurrences of many zones run side by side.  Every plane
-//                  rotation is logged; d ends up holding the eigenvalues.
-//   k_tql_apply    one workgroup per zone: applies the logged rotations to
-//                  the rows of Q in LDS, giving the eigenvectors U = Q Z.
-//   k_eigvec_epi   per zone: a = U diag(1/mu) U^T b (+ one refinement step
-//                  against M), v ~ U diag(mu^1/2) U^T 1, Y = U diag(mu^-1/4),
-//                  with mu = max(lambda, 1) (rrsqrt.F90:137-142, 176-177);
-//                  mu itself is kept beside a and v for the likelihood.
+//   k_tridiag      one workgroup per zone, matrix in registers (one tile
+//                  per thread): M = Q T Q^T with Householder reflectors
+//                  (lower), then Q formed explicitly by backward
+//                  accumulation.  Writes Q and (d, e).
+//   k_tql          one lane per zone (16 zones per wave): implicit QL with
+//                  Wilkinson shifts on (d, e) (Golub & Van Loan, 4th ed.,
+//                  section 8.3; Bowdler, Martin, Reinsch & Wilkinson 1968);
+//                  the sequential recurrences of many zones run side by side.
+//                  Every plane rotation is logged; d ends up holding the
+//                  eigenvalues.
+//   k_tql_apply    (N <= 64) one workgroup per zone: applies the logged
+//                  rotations to the rows of Q in LDS, giving U = Q Z.
+//   k_eigvec_epi   (N <= 64) per zone: a = U diag(1/mu) U^T b (+ one
+//                  refinement step against M), v ~ U diag(mu^1/2) U^T 1,
+//                  Y = U diag(mu^-1/4), with mu = max(lambda, 1)
+//                  (rrsqrt.F90:137-142, 176-177); mu itself is kept beside a
+//                  and v for the likelihood.
+//   k_tql_apply128 (N > 64) the rotations on rows held half in registers,
+//                  half in LDS, then the epilogue of k_eigvec_epi.
 #include <type_traits>
 
 #include "analysis.hpp"
@@ -893,7 +898,7 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   constexpr int ZW = 16;
   hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0,
                      A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
-  if (NP == 128) {  // the epilogue runs at the end of the apply kernel
+  if constexpr (NP == 128) {  // the epilogue runs at the end of the apply kernel
     hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(128), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
                        A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,
                        A.vec, A.ampl, A.err);

@@ -94,3 +94,33 @@ def test_mixed_empty_zones_keep_forecast():
     g = Geometry(P.zone_ptr, P.zpos, P.corr_len, cut, P.ox, P.oy, metric=0, loctype=1)
     Ea = A.ensemble_analysis(g, H, P.Hshift, P.yo, P.rmse2, E32)
     np.testing.assert_allclose(Ea, E32, rtol=0, atol=2e-6 * np.abs(E32).max())
+
+
+@pytest.mark.parametrize("N", [100, 128])
+def test_mixed_deep_columns_vs_oracle(N):
+    """161-element zones: the FP32 update runs two strip rounds per wave."""
+    from oak_amd import synth
+    from oak_amd import assimilation as A
+    P = synth.column_ocean(4, 3, 40, N, 40, 3, seed_cfg=320 + N, hlen=3, vlen=4, L=2.0,
+                           cutoff=5.0)
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    E32 = np.asfortranarray(P.E, dtype=np.float32)
+    for infl, maxc in ((1.0, np.inf), (1.05, 0.1)):
+        Ea = A.ensemble_analysis(_geo(P), H, P.Hshift, P.yo, P.rmse2, E32, inflation=infl,
+                                 max_correction=maxc)
+        e = ensemble_errors(Ea, _oracle(P, infl, maxc), P.E)
+        assert e["mean_err"] < TOL_MIXED and e["spread_err"] < TOL_MIXED, (N, infl, e)
+
+
+def test_mixed_empty_zones_keep_forecast_n128():
+    """N = 128, no observation within reach: T = I in the fused transform."""
+    from oak_amd import synth
+    from oak_amd import assimilation as A
+    from oak_amd.assimilation import Geometry
+    P = synth.column_ocean(6, 5, 40, 128, 40, 3, seed_cfg=333, hlen=3, vlen=4, L=2.0, cutoff=5.0)
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    E32 = np.asfortranarray(P.E, dtype=np.float32)
+    cut = np.full_like(P.max_len, 1e-9)
+    g = Geometry(P.zone_ptr, P.zpos, P.corr_len, cut, P.ox, P.oy, metric=0, loctype=1)
+    Ea = A.ensemble_analysis(g, H, P.Hshift, P.yo, P.rmse2, E32)
+    np.testing.assert_allclose(Ea, E32, rtol=0, atol=2e-6 * np.abs(E32).max())

@@ -38,8 +38,14 @@ __device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s)
 // from it; the tiles give their partial sums of A v and their share of
 // v^T A v (so p.v needs no second reduction); p = tau A v is summed per row;
 // and A -= v w^T + w v^T (w = p - tau/2 (p.v) v) is applied to the tiles that
-// meet the trailing block.  Three barriers per step.  The reflectors go to the rows of qmat; Q is then
-// accumulated backwards in the same tile layout and overwrites them.
+// meet the trailing block.  Three barriers per step.  The reflectors go to
+// the rows of qmat; Q is then accumulated backwards in the same tile layout
+// and overwrites them.
+// Two stages (NP >= 32): once the trailing block is the lower-right half,
+// it is passed through LDS to tiles of T/2 over that half, so the late steps
+// (and the early steps of the backward Q accumulation, whose active block is
+// that half) run on all four waves with a quarter of the work per thread
+// instead of on wave 0 alone.
 // T and Q are written in reversed order (d'_k = d_{NP-1-k}, columns of Q
 // reversed): the reduction from the top leaves the large diagonal entries at
 // the top of T, and QL, which deflates at the top of each block, converges
@@ -49,12 +55,30 @@ __device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s)
 // ---------------------------------------------------------------------------
 template <int NP>
 struct TriLds {
-  double col[2][NP];       // column i / Householder vector v (double-buffered)
-  double pw[2][NP];        // p (then y during the Q accumulation)
-  double part[16][NP + 1]; // per-tile partial sums
-  double red[2][4];        // per-wave shares of v^T A v (double-buffered)
+  double col[2][NP];  // column i / Householder vector v (double-buffered)
+  double pw[2][NP];   // p (then y during the Q accumulation)
+  union {
+    double part[16][NP + 1];             // per-tile partial sums
+    double xfer[NP / 2][NP / 2 + 1];     // the trailing half between the stages
+  };
+  double red[2][4];   // per-wave shares of v^T A v (double-buffered)
   double tau[NP];
 };
+
+// Tile (tr, tc) of a 16 x 16 tile grid held by thread t, in decreasing order
+// of min(tr, tc): the tiles that meet a trailing block [i+1, n)^2 are always
+// threads 0 .. (16-k)^2 - 1 (k = the smallest active tile index), so whole
+// waves fall idle as the block shrinks instead of a few lanes in every wave.
+// Thread t lies on ring s = floor(sqrt t) (tiles with min index 15 - s),
+// position j = t - s^2: j <= s on the ring's row, j > s on its column.
+__device__ __forceinline__ void ring_tile(int t, int& tr, int& tc) {
+  int s = (int)sqrtf((float)t);
+  while (s * s > t) --s;
+  while ((s + 1) * (s + 1) <= t) ++s;
+  const int j = t - s * s, mn = 15 - s;
+  if (j <= s) { tr = mn; tc = mn + j; }
+  else { tr = mn + (j - s); tc = mn; }
+}
 
 template <int NP>
 __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __restrict__ sel_ptr,
@@ -62,193 +86,249 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
                                                  double* __restrict__ qmat,
                                                  double* __restrict__ dvec, double* __restrict__ evec,
                                                  int nzc) {
-  constexpr int T = NP / 16;
+  constexpr int T1 = NP / 16;                  // tile of the first stage (whole matrix)
+  constexpr bool SPLIT = NP >= 32;
+  constexpr int H = SPLIT ? NP / 2 : 0;        // origin of the second stage's block
+  constexpr int T2 = SPLIT ? T1 / 2 : T1;      // tile of the second stage
   __shared__ TriLds<NP> S;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  // Tiles in decreasing order of min(tile row, tile column): the tiles that
-  // meet the trailing block [i+1, NP)^2 are always threads 0 .. (16-k)^2 - 1
-  // (k = the smallest active tile index), so whole waves fall idle as the
-  // block shrinks instead of idling a few lanes in every wave.  Thread t
-  // lies on ring s = floor(sqrt t) (tiles with min index 15 - s), position
-  // j = t - s^2: j <= s on the ring's row, j > s on its column.
   int tr, tc;
-  {
-    int s = (int)sqrtf((float)t);
-    while (s * s > t) --s;
-    while ((s + 1) * (s + 1) <= t) ++s;
-    const int j = t - s * s, mn = 15 - s;
-    if (j <= s) { tr = mn; tc = mn + j; }
-    else { tr = mn + (j - s); tc = mn; }
-  }
-  const int r0 = tr * T, c0 = tc * T;
+  ring_tile(t, tr, tc);
+  const int r1 = tr * T1, c1 = tc * T1;         // first-stage tile origin
+  const int r2 = H + tr * T2, c2 = H + tc * T2; // second-stage tile origin
   const double* M = mat + (size_t)zl * NP * NP;
   double* V = qmat + (size_t)zl * NP * NP;
-  double a[T][T];
-#pragma unroll
-  for (int ii = 0; ii < T; ++ii)
-#pragma unroll
-    for (int jj = 0; jj < T; ++jj) a[ii][jj] = M[(r0 + ii) * NP + c0 + jj];
 
-  for (int i = 0; i < NP - 2; ++i) {
-    const int b = i & 1;
-    double* col = S.col[b];
-    double* pw = S.pw[b];
-    // (1) column i, all rows, from the 16 threads of tile column i / T
-    if (tc == i / T) {
-      const int j0 = i % T;
+  // Steps i0 .. i1-1 of the reduction on tiles of T at (r0, c0); tile
+  // column tc covers columns base + tc*T ...
+  auto reduce_steps = [&](auto& a, auto Tk, int r0, int c0, int base, int i0, int i1) {
+    constexpr int T = decltype(Tk)::value;
+    for (int i = i0; i < i1; ++i) {
+      const int b = i & 1;
+      double* col = S.col[b];
+      double* pw = S.pw[b];
+      // (1) column i, all rows of the block, from the 16 threads of its tile column
+      if (tc == (i - base) / T) {
+        const int j0 = (i - base) % T;
 #pragma unroll
-      for (int jj = 0; jj < T; ++jj)
-        if (jj == j0) {
+        for (int jj = 0; jj < T; ++jj)
+          if (jj == j0) {
 #pragma unroll
-          for (int ii = 0; ii < T; ++ii) col[r0 + ii] = a[ii][jj];
-        }
-    }
-    lds_barrier();
-    // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i].
-    // Every wave forms the scalars from the same LDS column (identical
-    // results), so no barrier is needed before v is used.
-    double tau, scal;
-    {
-      double s = 0;
-      for (int r = i + 2 + lane; r < NP; r += 64) s = fma(col[r], col[r], s);
-      s = wave_sum_d(s);
-      const double alpha = col[i + 1];
-      double beta = alpha;
-      tau = 0.0;
-      scal = 0.0;
-      if (s > 0.0) {
-        const double nrm = sqrt(fma(alpha, alpha, s));
-        beta = alpha >= 0.0 ? -nrm : nrm;
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
+            for (int ii = 0; ii < T; ++ii) col[r0 + ii] = a[ii][jj];
+          }
       }
-      if (wave == 0) {
-        for (int r = lane; r < NP; r += 64)
-          V[(size_t)i * NP + r] = r <= i ? 0.0 : (r == i + 1 ? 1.0 : col[r] * scal);
-        if (lane == 0) {
-          dvec[(size_t)(NP - 1 - i) * nzc + zl] = col[i];
-          evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
-          S.tau[i] = tau;
+      lds_barrier();
+      // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i].
+      // Every wave forms the scalars from the same LDS column (identical
+      // results), so no barrier is needed before v is used.
+      double tau, scal;
+      {
+        double s = 0;
+        for (int r = i + 2 + lane; r < NP; r += 64) s = fma(col[r], col[r], s);
+        s = wave_sum_d(s);
+        const double alpha = col[i + 1];
+        double beta = alpha;
+        tau = 0.0;
+        scal = 0.0;
+        if (s > 0.0) {
+          const double nrm = sqrt(fma(alpha, alpha, s));
+          beta = alpha >= 0.0 ? -nrm : nrm;
+          tau = (beta - alpha) / beta;
+          scal = 1.0 / (alpha - beta);
+        }
+        if (wave == 0) {
+          for (int r = lane; r < NP; r += 64)
+            V[(size_t)i * NP + r] = r <= i ? 0.0 : (r == i + 1 ? 1.0 : col[r] * scal);
+          if (lane == 0) {
+            dvec[(size_t)(NP - 1 - i) * nzc + zl] = col[i];
+            evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
+            S.tau[i] = tau;
+          }
         }
       }
+      // (3) per-tile partial sums of A v and the tile's share of v^T A v.
+      // The column entries are loaded unconditionally (all in one LDS batch)
+      // and v is formed from them with selects.
+      const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
+      double vR[T], vC[T];
+#pragma unroll
+      for (int jj = 0; jj < T; ++jj) vC[jj] = col[c0 + jj];
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii) vR[ii] = col[r0 + ii];
+      auto vsel = [&](int r, double x) {
+        const double v = r == i + 1 ? 1.0 : x * scal;
+        return (act && r > i) ? v : 0.0;
+      };
+#pragma unroll
+      for (int jj = 0; jj < T; ++jj) vC[jj] = vsel(c0 + jj, vC[jj]);
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii) vR[ii] = vsel(r0 + ii, vR[ii]);
+      double qf = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < T; ++ii) {
+        double s = 0;
+        if (act) {
+#pragma unroll
+          for (int jj = 0; jj < T; ++jj) s = fma(a[ii][jj], vC[jj], s);
+        }
+        S.part[tc][r0 + ii] = s;
+        qf = fma(vR[ii], s, qf);
+      }
+      qf = wave_sum_d(qf);
+      if (lane == 0) S.red[b][wave] = qf;
+      lds_barrier();
+      // (4) p = tau A v; K = -tau/2 p^T v = -tau^2/2 v^T A v.  Rows above the
+      // block (second stage) read stale sums and get p = 0 (they are <= i).
+      if (t < NP) {
+        double s = 0;
+#pragma unroll
+        for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
+        pw[t] = t > i ? tau * s : 0.0;
+      }
+      lds_barrier();
+      // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
+      if (act) {
+        const double vav = ((S.red[b][0] + S.red[b][1]) + S.red[b][2]) + S.red[b][3];
+        const double K = -0.5 * tau * tau * vav;
+        double wR[T], wC[T];
+#pragma unroll
+        for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
+#pragma unroll
+        for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[c0 + jj]);
+#pragma unroll
+        for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < T; ++jj)
+            a[ii][jj] = fma(-vR[ii], wC[jj], fma(-wR[ii], vC[jj], a[ii][jj]));
+      }
     }
-    // (3) per-tile partial sums of A v and the tile's share of v^T A v.
-    // The column entries are loaded unconditionally (all in one LDS batch)
-    // and v is formed from them with selects.
-    const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
-    double vR[T], vC[T];
+  };
+
+  // Backward accumulation Q <- H_k Q for k = k_hi .. k_lo on tiles of T at
+  // (r0, c0); the reflector rows are read one step ahead into vnext.
+  double vnext = 0.0;
+  auto q_steps = [&](auto& q, auto Tk, int r0, int c0, int k_hi, int k_lo) {
+    constexpr int T = decltype(Tk)::value;
+    for (int k = k_hi; k >= k_lo; --k) {
+      const int b = k & 1;
+      double* vb = S.col[b];
+      double* yb = S.pw[b];
+      if (t < NP) vb[t] = vnext;
+      if (k > 0 && t < NP) vnext = V[(size_t)(k - 1) * NP + t];  // prefetch
+      lds_barrier();
+      const bool act = (r0 + T - 1 > k) && (c0 + T - 1 > k);
+      double vR[T];
 #pragma unroll
-    for (int jj = 0; jj < T; ++jj) vC[jj] = col[c0 + jj];
+      for (int ii = 0; ii < T; ++ii) vR[ii] = act ? vb[r0 + ii] : 0.0;
 #pragma unroll
-    for (int ii = 0; ii < T; ++ii) vR[ii] = col[r0 + ii];
-    auto vsel = [&](int r, double x) {
-      const double v = r == i + 1 ? 1.0 : x * scal;
-      return (act && r > i) ? v : 0.0;
-    };
+      for (int jj = 0; jj < T; ++jj) {
+        double s = 0;
+        if (act) {
 #pragma unroll
-    for (int jj = 0; jj < T; ++jj) vC[jj] = vsel(c0 + jj, vC[jj]);
+          for (int ii = 0; ii < T; ++ii) s = fma(vR[ii], q[ii][jj], s);
+        }
+        S.part[tr][c0 + jj] = s;
+      }
+      lds_barrier();
+      if (t < NP) {
+        double s = 0;
 #pragma unroll
-    for (int ii = 0; ii < T; ++ii) vR[ii] = vsel(r0 + ii, vR[ii]);
-    double qf = 0.0;
-#pragma unroll
-    for (int ii = 0; ii < T; ++ii) {
-      double s = 0;
+        for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
+        yb[t] = S.tau[k] * s;
+      }
+      lds_barrier();
       if (act) {
 #pragma unroll
-        for (int jj = 0; jj < T; ++jj) s = fma(a[ii][jj], vC[jj], s);
+        for (int ii = 0; ii < T; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < T; ++jj) q[ii][jj] = fma(-vR[ii], yb[c0 + jj], q[ii][jj]);
       }
-      S.part[tc][r0 + ii] = s;
-      qf = fma(vR[ii], s, qf);
     }
-    qf = wave_sum_d(qf);
-    if (lane == 0) S.red[b][wave] = qf;
-    lds_barrier();
-    // (4) p = tau A v; K = -tau/2 p^T v = -tau^2/2 v^T A v
-    if (t < NP) {
-      double s = 0;
+  };
+
+  // ---- reduction ----
+  double a2[T2][T2];
+  {
+    double a[T1][T1];
 #pragma unroll
-      for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
-      pw[t] = t > i ? tau * s : 0.0;
-    }
-    lds_barrier();
-    // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
-    if (act) {
-      const double vav = ((S.red[b][0] + S.red[b][1]) + S.red[b][2]) + S.red[b][3];
-      const double K = -0.5 * tau * tau * vav;
-      double wR[T], wC[T];
+    for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
-      for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
+      for (int jj = 0; jj < T1; ++jj) a[ii][jj] = M[(r1 + ii) * NP + c1 + jj];
+    if constexpr (SPLIT) {
+      reduce_steps(a, std::integral_constant<int, T1>{}, r1, c1, 0, 0, H);
+      // A[H.., H..] to the second-stage tiles (the sums in part are dead:
+      // every thread passed the barrier after step (4))
+      if (r1 >= H && c1 >= H) {
 #pragma unroll
-      for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[c0 + jj]);
+        for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
-      for (int ii = 0; ii < T; ++ii)
+          for (int jj = 0; jj < T1; ++jj) S.xfer[r1 - H + ii][c1 - H + jj] = a[ii][jj];
+      }
+      lds_barrier();
 #pragma unroll
-        for (int jj = 0; jj < T; ++jj) a[ii][jj] -= fma(vR[ii], wC[jj], wR[ii] * vC[jj]);
+      for (int ii = 0; ii < T2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < T2; ++jj) a2[ii][jj] = S.xfer[r2 - H + ii][c2 - H + jj];
+    } else {
+#pragma unroll
+      for (int ii = 0; ii < T1; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < T1; ++jj) a2[ii][jj] = a[ii][jj];
     }
   }
+  // part (aliasing xfer) is written again only after the barrier of step (1)
+  reduce_steps(a2, std::integral_constant<int, T2>{}, r2, c2, H, H, NP - 2);
   // trailing 2 x 2 block
 #pragma unroll
-  for (int ii = 0; ii < T; ++ii)
+  for (int ii = 0; ii < T2; ++ii)
 #pragma unroll
-    for (int jj = 0; jj < T; ++jj) {
-      const int r = r0 + ii, c = c0 + jj;
-      if (c == NP - 2 && r == NP - 2) dvec[(size_t)1 * nzc + zl] = a[ii][jj];
-      if (c == NP - 2 && r == NP - 1) evec[(size_t)0 * nzc + zl] = a[ii][jj];
-      if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a[ii][jj];
+    for (int jj = 0; jj < T2; ++jj) {
+      const int r = r2 + ii, c = c2 + jj;
+      if (c == NP - 2 && r == NP - 2) dvec[(size_t)1 * nzc + zl] = a2[ii][jj];
+      if (c == NP - 2 && r == NP - 1) evec[(size_t)0 * nzc + zl] = a2[ii][jj];
+      if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a2[ii][jj];
     }
   if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
 
   // ---- Q = H_0 H_1 ... H_{NP-3}, accumulated backwards: Q <- H_k Q ----
-  double q[T][T];
+  // While k >= H, Q differs from I only on the block [H.., H..].
+  double q2[T2][T2];
 #pragma unroll
-  for (int ii = 0; ii < T; ++ii)
+  for (int ii = 0; ii < T2; ++ii)
 #pragma unroll
-    for (int jj = 0; jj < T; ++jj) q[ii][jj] = (r0 + ii == c0 + jj) ? 1.0 : 0.0;
+    for (int jj = 0; jj < T2; ++jj) q2[ii][jj] = (r2 + ii == c2 + jj) ? 1.0 : 0.0;
   __syncthreads();  // reflectors written by wave 0 are visible to the workgroup
-  double vnext = (t < NP) ? V[(size_t)(NP - 3) * NP + t] : 0.0;
-  for (int k = NP - 3; k >= 0; --k) {
-    const int b = k & 1;
-    double* vb = S.col[b];
-    double* yb = S.pw[b];
-    if (t < NP) vb[t] = vnext;
-    if (k > 0 && t < NP) vnext = V[(size_t)(k - 1) * NP + t];  // prefetch
+  if (t < NP) vnext = V[(size_t)(NP - 3) * NP + t];
+  q_steps(q2, std::integral_constant<int, T2>{}, r2, c2, NP - 3, H);
+  double q[T1][T1];
+  if constexpr (SPLIT) {
+    // the yb reads of the last step ended before its final barrier
+#pragma unroll
+    for (int ii = 0; ii < T2; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < T2; ++jj) S.xfer[r2 - H + ii][c2 - H + jj] = q2[ii][jj];
     lds_barrier();
-    const bool act = (r0 + T - 1 > k) && (c0 + T - 1 > k);
-    double vR[T];
 #pragma unroll
-    for (int ii = 0; ii < T; ++ii) vR[ii] = act ? vb[r0 + ii] : 0.0;
+    for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
-    for (int jj = 0; jj < T; ++jj) {
-      double s = 0;
-      if (act) {
+      for (int jj = 0; jj < T1; ++jj)
+        q[ii][jj] = (r1 >= H && c1 >= H) ? S.xfer[r1 - H + ii][c1 - H + jj]
+                                         : (r1 + ii == c1 + jj ? 1.0 : 0.0);
+    q_steps(q, std::integral_constant<int, T1>{}, r1, c1, H - 1, 0);
+  } else {
 #pragma unroll
-        for (int ii = 0; ii < T; ++ii) s = fma(vR[ii], q[ii][jj], s);
-      }
-      S.part[tr][c0 + jj] = s;
-    }
-    lds_barrier();
-    if (t < NP) {
-      double s = 0;
+    for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
-      for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
-      yb[t] = S.tau[k] * s;
-    }
-    lds_barrier();
-    if (act) {
-#pragma unroll
-      for (int ii = 0; ii < T; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < T; ++jj) q[ii][jj] = fma(-vR[ii], yb[c0 + jj], q[ii][jj]);
-    }
+      for (int jj = 0; jj < T1; ++jj) q[ii][jj] = q2[ii][jj];
   }
   lds_barrier();
 #pragma unroll
-  for (int ii = 0; ii < T; ++ii)
+  for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
-    for (int jj = 0; jj < T; ++jj) V[(size_t)(r0 + ii) * NP + (NP - 1 - (c0 + jj))] = q[ii][jj];
+    for (int jj = 0; jj < T1; ++jj) V[(size_t)(r1 + ii) * NP + (NP - 1 - (c1 + jj))] = q[ii][jj];
 }
 
 // ---------------------------------------------------------------------------

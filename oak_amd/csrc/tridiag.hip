@@ -41,11 +41,11 @@ __device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s)
 // meet the trailing block.  Three barriers per step.  The reflectors go to
 // the rows of qmat; Q is then accumulated backwards in the same tile layout
 // and overwrites them.
-// Two stages (NP >= 32): once the trailing block is the lower-right half,
-// it is passed through LDS to tiles of T/2 over that half, so the late steps
-// (and the early steps of the backward Q accumulation, whose active block is
-// that half) run on all four waves with a quarter of the work per thread
-// instead of on wave 0 alone.
+// Up to three stages: once the trailing block is the lower-right half, it is
+// passed through LDS to tiles of T/2 over that half, and at the last quarter
+// to tiles of T/4, so the late steps (and the early steps of the backward Q
+// accumulation, whose active block is as small) run on all four waves with a
+// quarter (a sixteenth) of the work per thread instead of on wave 0 alone.
 // T and Q are written in reversed order (d'_k = d_{NP-1-k}, columns of Q
 // reversed): the reduction from the top leaves the large diagonal entries at
 // the top of T, and QL, which deflates at the top of each block, converges
@@ -86,10 +86,13 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
                                                  double* __restrict__ qmat,
                                                  double* __restrict__ dvec, double* __restrict__ evec,
                                                  int nzc) {
-  constexpr int T1 = NP / 16;                  // tile of the first stage (whole matrix)
-  constexpr bool SPLIT = NP >= 32;
-  constexpr int H = SPLIT ? NP / 2 : 0;        // origin of the second stage's block
-  constexpr int T2 = SPLIT ? T1 / 2 : T1;      // tile of the second stage
+  // Stage s runs on tiles of Ts over the block [Hs, NP)^2 = 16 Ts rows:
+  // T1 = NP/16 over the whole matrix, then T2 = T1/2 and T3 = T2/2 (as far
+  // as the tiles go down to 1 x 1).
+  constexpr int T1 = NP / 16;
+  constexpr int T2 = T1 >= 2 ? T1 / 2 : T1;
+  constexpr int T3 = T2 >= 2 ? T2 / 2 : T2;
+  constexpr int H2 = NP - 16 * T2, H3 = NP - 16 * T3;
   __shared__ TriLds<NP> S;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
@@ -97,8 +100,9 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   int tr, tc;
   ring_tile(t, tr, tc);
-  const int r1 = tr * T1, c1 = tc * T1;         // first-stage tile origin
-  const int r2 = H + tr * T2, c2 = H + tc * T2; // second-stage tile origin
+  const int r1 = tr * T1, c1 = tc * T1;            // tile origins per stage
+  const int r2 = H2 + tr * T2, c2 = H2 + tc * T2;
+  const int r3 = H3 + tr * T3, c3 = H3 + tc * T3;
   const double* M = mat + (size_t)zl * NP * NP;
   double* V = qmat + (size_t)zl * NP * NP;
 
@@ -249,80 +253,103 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
     }
   };
 
+  // The trailing block [Hd, NP)^2 from tiles of Ts at (rs, cs) to tiles of
+  // Td at (rd, cd) through LDS; destination tiles outside the block are set
+  // to the identity.  The callers' last reads of part (aliasing xfer) ended
+  // before the last barrier of their loop, and part is written again only
+  // after the first barrier of the next one.
+  auto move_block = [&](auto& src, auto Ts, int rs, int cs, auto& dst, auto Td, int rd, int cd,
+                        int Hd) {
+    constexpr int TS = decltype(Ts)::value, TD = decltype(Td)::value;
+    if (rs >= Hd && cs >= Hd) {
+#pragma unroll
+      for (int ii = 0; ii < TS; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < TS; ++jj) S.xfer[rs - Hd + ii][cs - Hd + jj] = src[ii][jj];
+    }
+    lds_barrier();
+    const bool in = rd >= Hd && cd >= Hd;
+#pragma unroll
+    for (int ii = 0; ii < TD; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < TD; ++jj)
+        dst[ii][jj] = in ? S.xfer[rd - Hd + ii][cd - Hd + jj] : (rd + ii == cd + jj ? 1.0 : 0.0);
+  };
+  auto copy_tile = [&](auto& src, auto& dst, auto Tk) {
+    constexpr int TT = decltype(Tk)::value;
+#pragma unroll
+    for (int ii = 0; ii < TT; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < TT; ++jj) dst[ii][jj] = src[ii][jj];
+  };
+  using K1 = std::integral_constant<int, T1>;
+  using K2 = std::integral_constant<int, T2>;
+  using K3 = std::integral_constant<int, T3>;
+
   // ---- reduction ----
-  double a2[T2][T2];
+  double a3[T3][T3];
   {
-    double a[T1][T1];
-#pragma unroll
-    for (int ii = 0; ii < T1; ++ii)
-#pragma unroll
-      for (int jj = 0; jj < T1; ++jj) a[ii][jj] = M[(r1 + ii) * NP + c1 + jj];
-    if constexpr (SPLIT) {
-      reduce_steps(a, std::integral_constant<int, T1>{}, r1, c1, 0, 0, H);
-      // A[H.., H..] to the second-stage tiles (the sums in part are dead:
-      // every thread passed the barrier after step (4))
-      if (r1 >= H && c1 >= H) {
-#pragma unroll
-        for (int ii = 0; ii < T1; ++ii)
-#pragma unroll
-          for (int jj = 0; jj < T1; ++jj) S.xfer[r1 - H + ii][c1 - H + jj] = a[ii][jj];
-      }
-      lds_barrier();
-#pragma unroll
-      for (int ii = 0; ii < T2; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < T2; ++jj) a2[ii][jj] = S.xfer[r2 - H + ii][c2 - H + jj];
-    } else {
+    double a2[T2][T2];
+    {
+      double a[T1][T1];
 #pragma unroll
       for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < T1; ++jj) a2[ii][jj] = a[ii][jj];
+        for (int jj = 0; jj < T1; ++jj) a[ii][jj] = M[(r1 + ii) * NP + c1 + jj];
+      if constexpr (T2 < T1) {
+        reduce_steps(a, K1{}, r1, c1, 0, 0, H2);
+        move_block(a, K1{}, r1, c1, a2, K2{}, r2, c2, H2);
+      } else {
+        copy_tile(a, a2, K1{});
+      }
+    }
+    if constexpr (T3 < T2) {
+      reduce_steps(a2, K2{}, r2, c2, H2, H2, H3);
+      move_block(a2, K2{}, r2, c2, a3, K3{}, r3, c3, H3);
+    } else {
+      copy_tile(a2, a3, K2{});
     }
   }
-  // part (aliasing xfer) is written again only after the barrier of step (1)
-  reduce_steps(a2, std::integral_constant<int, T2>{}, r2, c2, H, H, NP - 2);
+  reduce_steps(a3, K3{}, r3, c3, H3, H3, NP - 2);
   // trailing 2 x 2 block
 #pragma unroll
-  for (int ii = 0; ii < T2; ++ii)
+  for (int ii = 0; ii < T3; ++ii)
 #pragma unroll
-    for (int jj = 0; jj < T2; ++jj) {
-      const int r = r2 + ii, c = c2 + jj;
-      if (c == NP - 2 && r == NP - 2) dvec[(size_t)1 * nzc + zl] = a2[ii][jj];
-      if (c == NP - 2 && r == NP - 1) evec[(size_t)0 * nzc + zl] = a2[ii][jj];
-      if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a2[ii][jj];
+    for (int jj = 0; jj < T3; ++jj) {
+      const int r = r3 + ii, c = c3 + jj;
+      if (c == NP - 2 && r == NP - 2) dvec[(size_t)1 * nzc + zl] = a3[ii][jj];
+      if (c == NP - 2 && r == NP - 1) evec[(size_t)0 * nzc + zl] = a3[ii][jj];
+      if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a3[ii][jj];
     }
   if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
 
   // ---- Q = H_0 H_1 ... H_{NP-3}, accumulated backwards: Q <- H_k Q ----
-  // While k >= H, Q differs from I only on the block [H.., H..].
-  double q2[T2][T2];
-#pragma unroll
-  for (int ii = 0; ii < T2; ++ii)
-#pragma unroll
-    for (int jj = 0; jj < T2; ++jj) q2[ii][jj] = (r2 + ii == c2 + jj) ? 1.0 : 0.0;
-  __syncthreads();  // reflectors written by wave 0 are visible to the workgroup
-  if (t < NP) vnext = V[(size_t)(NP - 3) * NP + t];
-  q_steps(q2, std::integral_constant<int, T2>{}, r2, c2, NP - 3, H);
+  // While k >= Hs, Q differs from I only on the block [Hs, NP)^2.
   double q[T1][T1];
-  if constexpr (SPLIT) {
-    // the yb reads of the last step ended before its final barrier
+  {
+    double q2[T2][T2];
+    {
+      double q3[T3][T3];
 #pragma unroll
-    for (int ii = 0; ii < T2; ++ii)
+      for (int ii = 0; ii < T3; ++ii)
 #pragma unroll
-      for (int jj = 0; jj < T2; ++jj) S.xfer[r2 - H + ii][c2 - H + jj] = q2[ii][jj];
-    lds_barrier();
-#pragma unroll
-    for (int ii = 0; ii < T1; ++ii)
-#pragma unroll
-      for (int jj = 0; jj < T1; ++jj)
-        q[ii][jj] = (r1 >= H && c1 >= H) ? S.xfer[r1 - H + ii][c1 - H + jj]
-                                         : (r1 + ii == c1 + jj ? 1.0 : 0.0);
-    q_steps(q, std::integral_constant<int, T1>{}, r1, c1, H - 1, 0);
-  } else {
-#pragma unroll
-    for (int ii = 0; ii < T1; ++ii)
-#pragma unroll
-      for (int jj = 0; jj < T1; ++jj) q[ii][jj] = q2[ii][jj];
+        for (int jj = 0; jj < T3; ++jj) q3[ii][jj] = (r3 + ii == c3 + jj) ? 1.0 : 0.0;
+      __syncthreads();  // reflectors written by wave 0 are visible to the workgroup
+      if (t < NP) vnext = V[(size_t)(NP - 3) * NP + t];
+      q_steps(q3, K3{}, r3, c3, NP - 3, H3);
+      if constexpr (T3 < T2) {
+        move_block(q3, K3{}, r3, c3, q2, K2{}, r2, c2, H3);
+        q_steps(q2, K2{}, r2, c2, H3 - 1, H2);
+      } else {
+        copy_tile(q3, q2, K3{});
+      }
+    }
+    if constexpr (T2 < T1) {
+      move_block(q2, K2{}, r2, c2, q, K1{}, r1, c1, H2);
+      q_steps(q, K1{}, r1, c1, H2 - 1, 0);
+    } else {
+      copy_tile(q2, q, K2{});
+    }
   }
   lds_barrier();
 #pragma unroll

@@ -205,7 +205,7 @@ def roofline(F, phase, mixed, eig_name, config):
             "unit": "TFLOP/s", "frac": ach / peak, "traffic": phase_traffic(kern, mixed, config)}
 
 
-PHASE_KERNELS = {"eig": ["k_tridiag", "k_tql", "k_tql_apply128"],
+PHASE_KERNELS = {"eig": ["k_tridiag", "k_formq128", "k_tql", "k_tql_apply128"],
                  "gram": ["k_gram128"], "update": ["k_xform128"]}
 
 
@@ -339,8 +339,8 @@ def main():
     g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
     print("# generated %s in %.1f s" % (desc, time.time() - t0), file=sys.stderr, flush=True)
     n, N = DP.E.shape
-    eig_name = (("k_tridiag+k_tql+k_tql_apply128 (Householder + implicit QL + eigenvector "
-                 "epilogue, FP64)" if N > 64 else
+    eig_name = (("k_tridiag+k_formq128+k_tql+k_tql_apply128 (Householder, Q in blocked WY form "
+                 "on FP64 MFMA, implicit QL, eigenvector epilogue; FP64)" if N > 64 else
                  "k_tridiag+k_tql+k_tql_apply+k_eigvec_epi (Householder + implicit QL, FP64)")
                 if args.eig == "ql" else "k_eig (one-sided Jacobi, FP64)")
     eig_id = OAKG_EIG_QL if args.eig == "ql" else OAKG_EIG_JACOBI

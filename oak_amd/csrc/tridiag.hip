@@ -144,8 +144,11 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
           scal = 1.0 / (alpha - beta);
         }
         if (wave == 0) {
+          // NP = 128: tau goes to the unused entry i of row i (read by k_formq128)
           for (int r = lane; r < NP; r += 64)
-            V[(size_t)i * NP + r] = r <= i ? 0.0 : (r == i + 1 ? 1.0 : col[r] * scal);
+            V[(size_t)i * NP + r] = r < i ? 0.0
+                                  : r == i ? (NP == 128 ? tau : 0.0)
+                                           : (r == i + 1 ? 1.0 : col[r] * scal);
           if (lane == 0) {
             dvec[(size_t)(NP - 1 - i) * nzc + zl] = col[i];
             evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
@@ -322,6 +325,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a3[ii][jj];
     }
   if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
+  if constexpr (NP == 128) return;  // Q is formed by k_formq128 on MFMA
 
   // ---- Q = H_0 H_1 ... H_{NP-3}, accumulated backwards: Q <- H_k Q ----
   // While k >= Hs, Q differs from I only on the block [Hs, NP)^2.
@@ -356,6 +360,150 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
     for (int jj = 0; jj < T1; ++jj) V[(size_t)(r1 + ii) * NP + (NP - 1 - (c1 + jj))] = q[ii][jj];
+}
+
+// ---------------------------------------------------------------------------
+// Q = H_0 H_1 ... H_125 for NP = 128 from the reflectors k_tridiag leaves in
+// the rows of qmat (row k: v_k, with tau_k in its otherwise zero entry k),
+// accumulated backwards in blocks of 16 reflectors in compact WY form
+// (LAPACK dlarft / dlarfb): H_k0 ... H_k0+15 = I - Vb Tb Vb^T with Tb upper
+// triangular, and Q <- Q - Vb (Tb (Vb^T Q)), all products on
+// v_mfma_f64_16x16x4f64.  Q stays in the accumulators: wave w holds tile
+// columns w and 7 - w (8 tiles of 16 x 16 each, so the waves get similar
+// work as the active block grows); block b touches only tiles (I, J) with
+// I, J >= b.  The D register kk of a tile is the B operand of the k-step
+// over its rows 4 kk .. 4 kk + 3, so the three products chain in registers.
+// The next block's reflectors are loaded while a block is applied.  Q is
+// written with its columns reversed, as k_tridiag writes it for NP < 128.
+// ---------------------------------------------------------------------------
+struct FormQLds {
+  static constexpr int LV = 17;  // padded row of Vb and Tb
+  double vb[2][128 * LV];        // Vb[r][j]: row r of reflector k0 + j
+  double tb[2][16 * LV];         // Tb
+  double g[16 * LV];             // Vb^T Vb
+  double tau[2][16];
+};
+
+__global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int64_t* __restrict__ sel_ptr,
+                                                  double* __restrict__ qmat) {
+  constexpr int NP = 128, NB = 8, LV = FormQLds::LV;
+  __shared__ FormQLds S;
+  const int zl = blockIdx.x;
+  const int64_t z = z0 + zl;
+  if (sel_ptr[z + 1] == sel_ptr[z]) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int lr = lane >> 4, lc = lane & 15;
+  double* V = qmat + (size_t)zl * NP * NP;
+  const int J0 = wave, J1 = 7 - wave;
+  d4 q[2][8];
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int I = 0; I < 8; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        q[jj][I][r] = (I == (jj ? J1 : J0) && lr + 4 * r == lc) ? 1.0 : 0.0;
+  // thread t moves reflector pj = t / 16 of a block, rows pr .. pr + 7
+  const int pj = t >> 4, pr = 8 * (t & 15);
+  double pf[8];
+  auto load_block = [&](int b) {
+    const int k = 16 * b + pj;
+    if (k < NP - 2) {
+      const double2* src = reinterpret_cast<const double2*>(V + (size_t)k * NP + pr);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double2 x = src[e];
+        pf[2 * e] = x.x;
+        pf[2 * e + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) pf[e] = 0.0;
+    }
+  };
+  load_block(NB - 1);
+  for (int b = NB - 1; b >= 0; --b) {
+    const int buf = b & 1;
+    double* vb = S.vb[buf];
+    double* tb = S.tb[buf];
+    {
+      const int k = 16 * b + pj;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int r = pr + e;
+        vb[r * LV + pj] = r == k ? 0.0 : pf[e];
+        if (r == k) S.tau[buf][pj] = pf[e];
+      }
+    }
+    if (b > 0) load_block(b - 1);
+    lds_barrier();
+    // (1) wave 0: G = Vb^T Vb (rows >= 16 b), then Tb column by column:
+    //     Tb[j][j] = tau_j, Tb[0:j, j] = -tau_j Tb[0:j, 0:j] G[0:j, j]
+    if (wave == 0) {
+      d4 g = {0.0, 0.0, 0.0, 0.0};
+      for (int kk = 4 * b; kk < 32; ++kk) {
+        const double x = vb[(4 * kk + lr) * LV + lc];
+        g = mfma_f64(x, x, g);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) S.g[(lr + 4 * r) * LV + lc] = g[r];
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 16) {
+        double tr[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          asm volatile("" ::: "memory");  // G column by column, not all loads at once
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < j; ++k) s = fma(tr[k], S.g[k * LV + j], s);
+          const double tj = S.tau[buf][j];
+          tr[j] = lane < j ? -tj * s : (lane == j ? tj : 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) tb[lane * LV + j] = tr[j];
+      }
+    }
+    // (2) W_J = Vb^T Q[:, J] for the wave's tile columns J >= b
+    d4 w[2];
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      w[jj] = d4{0.0, 0.0, 0.0, 0.0};
+      if ((jj ? J1 : J0) >= b) {
+#pragma unroll
+        for (int I = 0; I < 8; ++I)
+          if (I >= b) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+              w[jj] = mfma_f64(vb[(16 * I + 4 * kk + lr) * LV + lc], q[jj][I][kk], w[jj]);
+          }
+      }
+    }
+    lds_barrier();  // Tb
+    // (3) W'_J = -Tb W_J;  Q[I][J] += Vb[I rows] W'_J
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      if ((jj ? J1 : J0) >= b) {
+        d4 wp = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) wp = mfma_f64(-tb[lc * LV + 4 * kk + lr], w[jj][kk], wp);
+#pragma unroll
+        for (int I = 0; I < 8; ++I)
+          if (I >= b) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+              q[jj][I] = mfma_f64(vb[(16 * I + lc) * LV + 4 * kk + lr], wp[kk], q[jj][I]);
+          }
+      }
+    }
+  }
+  // every reflector row was read before the last barrier
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int I = 0; I < 8; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        V[(size_t)(16 * I + lr + 4 * r) * NP + (NP - 1 - (16 * (jj ? J1 : J0) + lc))] = q[jj][I][r];
 }
 
 // ---------------------------------------------------------------------------
@@ -443,12 +591,14 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
       double dnext2 = 0.0;  // new d_{i+2}
       int jmin = m;
       bool under = false;
+      // wait for the loads above here: left to the compiler, the wait lands
+      // in the loop header and runs on every step
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
       for (; i >= l; --i) {
-        double ein = 0.0, din = 0.0;
-        if (i > l) {  // operands of the next step, loaded ahead
-          ein = e[i - 1][lane];
-          din = d[i - 1][lane];
-        }
+        // operands of the next step, loaded ahead (unconditionally: a load
+        // under a branch is waited for at the join; unused when i == l)
+        const int im = i > 0 ? i - 1 : 0;
+        const double ein = e[im][lane], din = d[im][lane];
         const double f = s * ei, bb = c * ei;
         const double h = fma(f, f, g * g);
         if (h == 0.0) {  // exact zero: the block splits at i + 1
@@ -458,15 +608,18 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
           under = true;
           break;
         }
+        // r = sqrt(h), s' = f / r, c' = g / r; with g2 = d_{i+1} - p,
+        // (d_i - g2) s' + 2 c' bb = X / r, X formed while 1/r is computed
+        const double g2 = dip1 - p;
+        const double X = fma(di - g2, f, 2.0 * g * bb);
         const double ri = rsqrt_acc(h);
         const double r = h * ri;
         e[i + 1][lane] = r;
         s = f * ri;
         c = g * ri;
-        g = dip1 - p;
-        const double rr = fma(di - g, s, 2.0 * c * bb);
+        const double rr = X * ri;
         p = s * rr;
-        const double dn = g + p;
+        const double dn = g2 + p;
         d[i + 1][lane] = dn;
         g = fma(c, rr, -bb);
         lg[rlog_rec(pos)] = make_double2(c, s);
@@ -1002,6 +1155,8 @@ template <int NP>
 static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   hipLaunchKernelGGL(k_tridiag<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.mat, A.ymat,
                      A.dvec, A.evec, A.nzc);
+  if constexpr (NP == 128)
+    hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.ymat);
   constexpr int ZW = 16;
   hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0,
                      A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);

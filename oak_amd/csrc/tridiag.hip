@@ -518,14 +518,13 @@ __device__ __forceinline__ size_t rlog_off(int zl, int64_t cap) {
 }
 __device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) * 256 + (size_t)(k & 3); }
 
-// 1/sqrt(h) to full double precision: hardware estimate + two Newton steps
+// 1/sqrt(h) to full double precision: the hardware estimate y (relative
+// error below 2^-22) and one third-order correction, y (1 + e/2 + 3e^2/8)
+// with e = 1 - h y^2 (the neglected term 5e^3/16 is below 2^-64)
 __device__ __forceinline__ double rsqrt_acc(double h) {
-  double y = __builtin_amdgcn_rsq(h);
-  double hy = h * y;
-  y = fma(y * 0.5, fma(-hy, y, 1.0), y);
-  hy = h * y;
-  y = fma(y * 0.5, fma(-hy, y, 1.0), y);
-  return y;
+  const double y = __builtin_amdgcn_rsq(h);
+  const double e = fma(-h, y * y, 1.0);
+  return fma(y * e, fma(0.375, e, 0.5), y);
 }
 
 // Implicit QL with Wilkinson shifts on (d, e), one lane per zone (EISPACK

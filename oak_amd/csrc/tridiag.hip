@@ -31,6 +31,21 @@ namespace oakg {
 
 __device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s); }
 
+// 1/sqrt(h) to full double precision: the hardware estimate y (relative
+// error below 2^-22) and one third-order correction, y (1 + e/2 + 3e^2/8)
+// with e = 1 - h y^2 (the neglected term 5e^3/16 is below 2^-64)
+__device__ __forceinline__ double rsqrt_acc(double h) {
+  const double y = __builtin_amdgcn_rsq(h);
+  const double e = fma(-h, y * y, 1.0);
+  return fma(y * e, fma(0.375, e, 0.5), y);
+}
+// 1/x the same way: y (1 + e + e^2) with e = 1 - x y
+__device__ __forceinline__ double rcp_acc(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, y, 1.0);
+  return fma(fma(e, e, e), y, y);
+}
+
 // ---------------------------------------------------------------------------
 // Householder tridiagonalisation (lower, as dsytd2) with the matrix in
 // registers: thread t of 256 holds one T x T tile, T = NP/16.  Each step
@@ -55,7 +70,9 @@ __device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s)
 // ---------------------------------------------------------------------------
 template <int NP>
 struct TriLds {
-  double col[2][NP];  // column i / Householder vector v (double-buffered)
+  double colw[4][NP]; // each wave's copy of the current column (reduction)
+  double pub[2][NP];  // column i+1 before step i's update (double-buffered)
+  double col[2][NP];  // Householder vector v (Q accumulation, double-buffered)
   double pw[2][NP];   // p (then y during the Q accumulation)
   union {
     double part[16][NP + 1];             // per-tile partial sums
@@ -107,26 +124,21 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   double* V = qmat + (size_t)zl * NP * NP;
 
   // Steps i0 .. i1-1 of the reduction on tiles of T at (r0, c0); tile
-  // column tc covers columns base + tc*T ...
+  // column tc covers columns base + tc*T ...  Each wave keeps its own copy of
+  // the current column i in LDS (S.colw[wave], written and read by that wave
+  // only).  Step i publishes column i+1 as it stands before the step's
+  // update, and every wave then forms the updated column itself, so a step
+  // needs two barriers and the next reflector's scalars are computed while
+  // the tiles are still being updated.
+  double* col = S.colw[wave];
   auto reduce_steps = [&](auto& a, auto Tk, int r0, int c0, int base, int i0, int i1) {
     constexpr int T = decltype(Tk)::value;
     for (int i = i0; i < i1; ++i) {
       const int b = i & 1;
-      double* col = S.col[b];
       double* pw = S.pw[b];
-      // (1) column i, all rows of the block, from the 16 threads of its tile column
-      if (tc == (i - base) / T) {
-        const int j0 = (i - base) % T;
-#pragma unroll
-        for (int jj = 0; jj < T; ++jj)
-          if (jj == j0) {
-#pragma unroll
-            for (int ii = 0; ii < T; ++ii) col[r0 + ii] = a[ii][jj];
-          }
-      }
-      lds_barrier();
+      double* pub = S.pub[b];
       // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i].
-      // Every wave forms the scalars from the same LDS column (identical
+      // Every wave forms the scalars from its copy of the column (identical
       // results), so no barrier is needed before v is used.
       double tau, scal;
       {
@@ -138,10 +150,23 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
         tau = 0.0;
         scal = 0.0;
         if (s > 0.0) {
-          const double nrm = sqrt(fma(alpha, alpha, s));
-          beta = alpha >= 0.0 ? -nrm : nrm;
-          tau = (beta - alpha) / beta;
-          scal = 1.0 / (alpha - beta);
+          // beta = -sign(alpha) nrm, tau = (beta - alpha) / beta = 1 + |alpha| / nrm,
+          // 1 / (alpha - beta) = sign(alpha) / (|alpha| + nrm)
+          const double h = fma(alpha, alpha, s);
+          const double aa = fabs(alpha);
+          if (h > 0x1p-900 && h < 0x1p+900) {
+            const double rn = rsqrt_acc(h);
+            const double nrm = h * rn;
+            beta = alpha >= 0.0 ? -nrm : nrm;
+            tau = fma(aa, rn, 1.0);
+            const double rd = rcp_acc(aa + nrm);
+            scal = alpha >= 0.0 ? rd : -rd;
+          } else {
+            const double nrm = sqrt(h);
+            beta = alpha >= 0.0 ? -nrm : nrm;
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+          }
         }
         if (wave == 0) {
           // NP = 128: tau goes to the unused entry i of row i (read by k_formq128)
@@ -184,22 +209,35 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
         S.part[tc][r0 + ii] = s;
         qf = fma(vR[ii], s, qf);
       }
+      // column i+1 before this step's update, from the threads of its tile column
+      if (tc == (i + 1 - base) / T) {
+        const int j0 = (i + 1 - base) % T;
+#pragma unroll
+        for (int jj = 0; jj < T; ++jj)
+          if (jj == j0) {
+#pragma unroll
+            for (int ii = 0; ii < T; ++ii) pub[r0 + ii] = a[ii][jj];
+          }
+      }
       qf = wave_sum_d(qf);
       if (lane == 0) S.red[b][wave] = qf;
       lds_barrier();
       // (4) p = tau A v; K = -tau/2 p^T v = -tau^2/2 v^T A v.  Rows above the
       // block (second stage) read stale sums and get p = 0 (they are <= i).
       if (t < NP) {
-        double s = 0;
+        double s4[4];
 #pragma unroll
-        for (int qq = 0; qq < 16; ++qq) s += S.part[qq][t];
+        for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][t];
+#pragma unroll
+        for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][t];
+        const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         pw[t] = t > i ? tau * s : 0.0;
       }
       lds_barrier();
       // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
+      const double vav = ((S.red[b][0] + S.red[b][1]) + S.red[b][2]) + S.red[b][3];
+      const double K = -0.5 * tau * tau * vav;
       if (act) {
-        const double vav = ((S.red[b][0] + S.red[b][1]) + S.red[b][2]) + S.red[b][3];
-        const double K = -0.5 * tau * tau * vav;
         double wR[T], wC[T];
 #pragma unroll
         for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
@@ -211,6 +249,22 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
           for (int jj = 0; jj < T; ++jj)
             a[ii][jj] = fma(-vR[ii], wC[jj], fma(-wR[ii], vC[jj], a[ii][jj]));
       }
+      // column i+1 after the update (the same operations as on the tiles:
+      // v_{i+1} = 1, w_{i+1} = p_{i+1} + K), rows r = lane, lane + 64
+      {
+        const double w1 = fma(K, 1.0, pw[i + 1]);
+#pragma unroll
+        for (int h = 0; h < (NP + 63) / 64; ++h) {
+          const int r = lane + 64 * h;
+          if (r < NP) {
+            const double v = r == i + 1 ? 1.0 : col[r] * scal;
+            const double w = fma(K, v, pw[r]);
+            const double nc = fma(-v, w1, fma(-w, 1.0, pub[r]));
+            col[r] = r > i ? nc : 0.0;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
     }
   };
 
@@ -277,6 +331,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
 #pragma unroll
       for (int jj = 0; jj < TD; ++jj)
         dst[ii][jj] = in ? S.xfer[rd - Hd + ii][cd - Hd + jj] : (rd + ii == cd + jj ? 1.0 : 0.0);
+    lds_barrier();  // xfer read before part is written again
   };
   auto copy_tile = [&](auto& src, auto& dst, auto Tk) {
     constexpr int TT = decltype(Tk)::value;
@@ -299,6 +354,8 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
         for (int jj = 0; jj < T1; ++jj) a[ii][jj] = M[(r1 + ii) * NP + c1 + jj];
+      for (int r = lane; r < NP; r += 64) col[r] = M[(size_t)r * NP];  // column 0
+      __builtin_amdgcn_wave_barrier();
       if constexpr (T2 < T1) {
         reduce_steps(a, K1{}, r1, c1, 0, 0, H2);
         move_block(a, K1{}, r1, c1, a2, K2{}, r2, c2, H2);
@@ -518,14 +575,6 @@ __device__ __forceinline__ size_t rlog_off(int zl, int64_t cap) {
 }
 __device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) * 256 + (size_t)(k & 3); }
 
-// 1/sqrt(h) to full double precision: the hardware estimate y (relative
-// error below 2^-22) and one third-order correction, y (1 + e/2 + 3e^2/8)
-// with e = 1 - h y^2 (the neglected term 5e^3/16 is below 2^-64)
-__device__ __forceinline__ double rsqrt_acc(double h) {
-  const double y = __builtin_amdgcn_rsq(h);
-  const double e = fma(-h, y * y, 1.0);
-  return fma(y * e, fma(0.375, e, 0.5), y);
-}
 
 // Implicit QL with Wilkinson shifts on (d, e), one lane per zone (EISPACK
 // tql2 ordering: eigenvalue l converges at the top of the block l..m).  The

@@ -168,36 +168,36 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
             scal = 1.0 / (alpha - beta);
           }
         }
+        const double di = col[i];
+        // the wave's copy of the column becomes v (0 up to row i, 1 at i+1)
+#pragma unroll
+        for (int h = 0; h < (NP + 63) / 64; ++h) {
+          const int r = lane + 64 * h;
+          if (r < NP) {
+            const double x = col[r];
+            col[r] = r > i + 1 ? x * scal : (r == i + 1 ? 1.0 : 0.0);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
         if (wave == 0) {
           // NP = 128: tau goes to the unused entry i of row i (read by k_formq128)
           for (int r = lane; r < NP; r += 64)
-            V[(size_t)i * NP + r] = r < i ? 0.0
-                                  : r == i ? (NP == 128 ? tau : 0.0)
-                                           : (r == i + 1 ? 1.0 : col[r] * scal);
+            V[(size_t)i * NP + r] = r == i ? (NP == 128 ? tau : 0.0) : col[r];
           if (lane == 0) {
-            dvec[(size_t)(NP - 1 - i) * nzc + zl] = col[i];
+            dvec[(size_t)(NP - 1 - i) * nzc + zl] = di;
             evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
             S.tau[i] = tau;
           }
         }
       }
-      // (3) per-tile partial sums of A v and the tile's share of v^T A v.
-      // The column entries are loaded unconditionally (all in one LDS batch)
-      // and v is formed from them with selects.
+      // (3) per-tile partial sums of A v and the tile's share of v^T A v
+      // (an inactive tile has zero sums; its v entries are then not used)
       const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
       double vR[T], vC[T];
 #pragma unroll
       for (int jj = 0; jj < T; ++jj) vC[jj] = col[c0 + jj];
 #pragma unroll
       for (int ii = 0; ii < T; ++ii) vR[ii] = col[r0 + ii];
-      auto vsel = [&](int r, double x) {
-        const double v = r == i + 1 ? 1.0 : x * scal;
-        return (act && r > i) ? v : 0.0;
-      };
-#pragma unroll
-      for (int jj = 0; jj < T; ++jj) vC[jj] = vsel(c0 + jj, vC[jj]);
-#pragma unroll
-      for (int ii = 0; ii < T; ++ii) vR[ii] = vsel(r0 + ii, vR[ii]);
       double qf = 0.0;
 #pragma unroll
       for (int ii = 0; ii < T; ++ii) {
@@ -223,15 +223,18 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       if (lane == 0) S.red[b][wave] = qf;
       lds_barrier();
       // (4) p = tau A v; K = -tau/2 p^T v = -tau^2/2 v^T A v.  Rows above the
-      // block (second stage) read stale sums and get p = 0 (they are <= i).
-      if (t < NP) {
+      // block (later stages) read stale sums and get p = 0 (they are <= i).
+      // The last NP threads do it: the ring order keeps their waves the
+      // least busy with tiles.
+      if (t >= 256 - NP) {
+        const int row = t - (256 - NP);
         double s4[4];
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][t];
+        for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][row];
 #pragma unroll
-        for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][t];
+        for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][row];
         const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        pw[t] = t > i ? tau * s : 0.0;
+        pw[row] = row > i ? tau * s : 0.0;
       }
       lds_barrier();
       // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
         for (int h = 0; h < (NP + 63) / 64; ++h) {
           const int r = lane + 64 * h;
           if (r < NP) {
-            const double v = r == i + 1 ? 1.0 : col[r] * scal;
+            const double v = col[r];
             const double w = fma(K, v, pw[r]);
             const double nc = fma(-v, w1, fma(-w, 1.0, pub[r]));
             col[r] = r > i ? nc : 0.0;

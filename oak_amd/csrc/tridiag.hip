@@ -124,12 +124,17 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   double* V = qmat + (size_t)zl * NP * NP;
 
   // Steps i0 .. i1-1 of the reduction on tiles of T at (r0, c0); tile
-  // column tc covers columns base + tc*T ...  Each wave keeps its own copy of
-  // the current column i in LDS (S.colw[wave], written and read by that wave
-  // only).  Step i publishes column i+1 as it stands before the step's
-  // update, and every wave then forms the updated column itself, so a step
-  // needs two barriers and the next reflector's scalars are computed while
-  // the tiles are still being updated.
+  // column tc covers columns base + tc*T ...  Every wave carries the current
+  // column i in registers (x[h] = A[lane + 64 h][i]) and forms the
+  // reflector's scalars from it; v goes to the wave's own LDS copy
+  // (S.colw[wave]) for the tiles.  Step i publishes column i+1 as it stands
+  // before the step's update, and every wave then forms the updated column
+  // itself, so a step needs two barriers.  The threads that sum the rows of
+  // A v (the last NP, whose waves hold the fewest active tiles in the ring
+  // order) also form v^T p; the last wave writes the reflector out.
+  constexpr int NH = (NP + 63) / 64;
+  constexpr int PW0 = (256 - NP) / 64;  // first wave of the row sums
+  double x[NH];
   double* col = S.colw[wave];
   auto reduce_steps = [&](auto& a, auto Tk, int r0, int c0, int base, int i0, int i1) {
     constexpr int T = decltype(Tk)::value;
@@ -138,51 +143,56 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       double* pw = S.pw[b];
       double* pub = S.pub[b];
       // (2) reflector H = I - tau v v^T with H x = beta e_1, x = A[i+1.., i].
-      // Every wave forms the scalars from its copy of the column (identical
-      // results), so no barrier is needed before v is used.
-      double tau, scal;
+      // Every wave forms the scalars (identical results), so no barrier is
+      // needed before v is used.
+      double tau = 0.0, scal = 0.0, beta;
+      double v[NH];
       {
-        double s = 0;
-        for (int r = i + 2 + lane; r < NP; r += 64) s = fma(col[r], col[r], s);
+        double s = 0.0;
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+          const int r = lane + 64 * h;
+          if (r >= i + 2 && r < NP) s = fma(x[h], x[h], s);
+        }
         s = wave_sum_d(s);
-        const double alpha = col[i + 1];
-        double beta = alpha;
-        tau = 0.0;
-        scal = 0.0;
+        const int ia = i + 1, id = i;
+        const double alpha = readlane_d((ia >> 6) ? x[NH - 1] : x[0], ia & 63);
+        const double di = readlane_d((id >> 6) ? x[NH - 1] : x[0], id & 63);
+        beta = alpha;
         if (s > 0.0) {
           // beta = -sign(alpha) nrm, tau = (beta - alpha) / beta = 1 + |alpha| / nrm,
           // 1 / (alpha - beta) = sign(alpha) / (|alpha| + nrm)
-          const double h = fma(alpha, alpha, s);
+          const double hh = fma(alpha, alpha, s);
           const double aa = fabs(alpha);
-          if (h > 0x1p-900 && h < 0x1p+900) {
-            const double rn = rsqrt_acc(h);
-            const double nrm = h * rn;
+          if (hh > 0x1p-900 && hh < 0x1p+900) {
+            const double rn = rsqrt_acc(hh);
+            const double nrm = hh * rn;
             beta = alpha >= 0.0 ? -nrm : nrm;
             tau = fma(aa, rn, 1.0);
             const double rd = rcp_acc(aa + nrm);
             scal = alpha >= 0.0 ? rd : -rd;
           } else {
-            const double nrm = sqrt(h);
+            const double nrm = sqrt(hh);
             beta = alpha >= 0.0 ? -nrm : nrm;
             tau = (beta - alpha) / beta;
             scal = 1.0 / (alpha - beta);
           }
         }
-        const double di = col[i];
-        // the wave's copy of the column becomes v (0 up to row i, 1 at i+1)
+        // v: 0 up to row i, 1 at i+1, x * scal below
 #pragma unroll
-        for (int h = 0; h < (NP + 63) / 64; ++h) {
+        for (int h = 0; h < NH; ++h) {
           const int r = lane + 64 * h;
-          if (r < NP) {
-            const double x = col[r];
-            col[r] = r > i + 1 ? x * scal : (r == i + 1 ? 1.0 : 0.0);
-          }
+          v[h] = r > i + 1 ? x[h] * scal : (r == i + 1 ? 1.0 : 0.0);
+          if (r < NP) col[r] = v[h];
         }
         __builtin_amdgcn_wave_barrier();
-        if (wave == 0) {
+        if (wave == 3) {
           // NP = 128: tau goes to the unused entry i of row i (read by k_formq128)
-          for (int r = lane; r < NP; r += 64)
-            V[(size_t)i * NP + r] = r == i ? (NP == 128 ? tau : 0.0) : col[r];
+#pragma unroll
+          for (int h = 0; h < NH; ++h) {
+            const int r = lane + 64 * h;
+            if (r < NP) V[(size_t)i * NP + r] = r == i ? (NP == 128 ? tau : 0.0) : v[h];
+          }
           if (lane == 0) {
             dvec[(size_t)(NP - 1 - i) * nzc + zl] = di;
             evec[(size_t)(NP - 2 - i) * nzc + zl] = beta;
@@ -190,24 +200,22 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
           }
         }
       }
-      // (3) per-tile partial sums of A v and the tile's share of v^T A v
-      // (an inactive tile has zero sums; its v entries are then not used)
+      // (3) per-tile partial sums of A v (an inactive tile has zero sums;
+      // its v entries are then not used)
       const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
       double vR[T], vC[T];
 #pragma unroll
       for (int jj = 0; jj < T; ++jj) vC[jj] = col[c0 + jj];
 #pragma unroll
       for (int ii = 0; ii < T; ++ii) vR[ii] = col[r0 + ii];
-      double qf = 0.0;
 #pragma unroll
       for (int ii = 0; ii < T; ++ii) {
-        double s = 0;
+        double ps = 0;
         if (act) {
 #pragma unroll
-          for (int jj = 0; jj < T; ++jj) s = fma(a[ii][jj], vC[jj], s);
+          for (int jj = 0; jj < T; ++jj) ps = fma(a[ii][jj], vC[jj], ps);
         }
-        S.part[tc][r0 + ii] = s;
-        qf = fma(vR[ii], s, qf);
+        S.part[tc][r0 + ii] = ps;
       }
       // column i+1 before this step's update, from the threads of its tile column
       if (tc == (i + 1 - base) / T) {
@@ -219,27 +227,33 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
             for (int ii = 0; ii < T; ++ii) pub[r0 + ii] = a[ii][jj];
           }
       }
-      qf = wave_sum_d(qf);
-      if (lane == 0) S.red[b][wave] = qf;
       lds_barrier();
-      // (4) p = tau A v; K = -tau/2 p^T v = -tau^2/2 v^T A v.  Rows above the
-      // block (later stages) read stale sums and get p = 0 (they are <= i).
-      // The last NP threads do it: the ring order keeps their waves the
-      // least busy with tiles.
-      if (t >= 256 - NP) {
+      // (4) p = tau A v and the waves' shares of v^T p.  Rows above the block
+      // (later stages) read stale sums and get p = 0 (they are <= i).
+      if (wave >= PW0) {
         const int row = t - (256 - NP);
-        double s4[4];
+        double vp = 0.0;
+        if (row >= 0) {
+          double s4[4];
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][row];
+          for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][row];
 #pragma unroll
-        for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][row];
-        const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        pw[row] = row > i ? tau * s : 0.0;
+          for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][row];
+          const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+          const double pr = row > i ? tau * s : 0.0;
+          pw[row] = pr;
+          vp = col[row] * pr;
+        }
+        vp = wave_sum_d(vp);
+        if (lane == 0) S.red[b][wave] = vp;
       }
       lds_barrier();
-      // (5) A -= v w^T + w v^T on the tiles that meet the trailing block
-      const double vav = ((S.red[b][0] + S.red[b][1]) + S.red[b][2]) + S.red[b][3];
-      const double K = -0.5 * tau * tau * vav;
+      // (5) A -= v w^T + w v^T on the tiles that meet the trailing block;
+      // w = p + K v, K = -tau/2 v^T p
+      double vpt = 0.0;
+#pragma unroll
+      for (int w = PW0; w < 4; ++w) vpt += S.red[b][w];
+      const double K = -0.5 * tau * vpt;
       if (act) {
         double wR[T], wC[T];
 #pragma unroll
@@ -253,21 +267,19 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
             a[ii][jj] = fma(-vR[ii], wC[jj], fma(-wR[ii], vC[jj], a[ii][jj]));
       }
       // column i+1 after the update (the same operations as on the tiles:
-      // v_{i+1} = 1, w_{i+1} = p_{i+1} + K), rows r = lane, lane + 64
+      // v_{i+1} = 1, w_{i+1} = p_{i+1} + K)
       {
         const double w1 = fma(K, 1.0, pw[i + 1]);
 #pragma unroll
-        for (int h = 0; h < (NP + 63) / 64; ++h) {
+        for (int h = 0; h < NH; ++h) {
           const int r = lane + 64 * h;
           if (r < NP) {
-            const double v = col[r];
-            const double w = fma(K, v, pw[r]);
-            const double nc = fma(-v, w1, fma(-w, 1.0, pub[r]));
-            col[r] = r > i ? nc : 0.0;
+            const double w = fma(K, v[h], pw[r]);
+            const double nc = fma(-v[h], w1, fma(-w, 1.0, pub[r]));
+            x[h] = r > i ? nc : 0.0;
           }
         }
       }
-      __builtin_amdgcn_wave_barrier();
     }
   };
 
@@ -357,8 +369,11 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       for (int ii = 0; ii < T1; ++ii)
 #pragma unroll
         for (int jj = 0; jj < T1; ++jj) a[ii][jj] = M[(r1 + ii) * NP + c1 + jj];
-      for (int r = lane; r < NP; r += 64) col[r] = M[(size_t)r * NP];  // column 0
-      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {  // column 0
+      const int r = lane + 64 * h;
+      x[h] = r < NP ? M[(size_t)r * NP] : 0.0;
+    }
       if constexpr (T2 < T1) {
         reduce_steps(a, K1{}, r1, c1, 0, 0, H2);
         move_block(a, K1{}, r1, c1, a2, K2{}, r2, c2, H2);

@@ -68,14 +68,21 @@ __device__ __forceinline__ double rcp_acc(double x) {
 // observed zones; LAPACK's dsteqr switches to QR there).  Reversed, the
 // small entries come first and QL needs a few sweeps per eigenvalue.
 // ---------------------------------------------------------------------------
+// The reduction indexes its row vectors with one pad entry per 8 rows,
+// tri_rx(r) = r + r / 8: the tiles of the first stage start every 8 rows,
+// and without the pad the reads of v and p by the 16 tile rows of a wave
+// fall four to a bank.
+__host__ __device__ constexpr int tri_rx(int r) { return r + (r >> 3); }
+
 template <int NP>
 struct TriLds {
-  double colw[4][NP]; // each wave's copy of the current column (reduction)
-  double pub[2][NP];  // column i+1 before step i's update (double-buffered)
+  static constexpr int PR = NP + NP / 8;  // padded length of a row vector
+  double colw[4][PR]; // each wave's copy of v (reduction)
+  double pub[2][PR];  // column i+1 before step i's update (double-buffered)
   double col[2][NP];  // Householder vector v (Q accumulation, double-buffered)
-  double pw[2][NP];   // p (then y during the Q accumulation)
+  double pw[2][PR];   // p (then y during the Q accumulation)
   union {
-    double part[16][NP + 1];             // per-tile partial sums
+    double part[16][PR + 1];             // per-tile partial sums
     double xfer[NP / 2][NP / 2 + 1];     // the trailing half between the stages
   };
   double red[2][4];   // per-wave shares of v^T A v (double-buffered)
@@ -138,6 +145,8 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   double* col = S.colw[wave];
   auto reduce_steps = [&](auto& a, auto Tk, int r0, int c0, int base, int i0, int i1) {
     constexpr int T = decltype(Tk)::value;
+    // a tile's T rows (columns) lie in one group of 8: tri_rx(r0 + ii) = R0 + ii
+    const int R0 = tri_rx(r0), C0 = tri_rx(c0);
     for (int i = i0; i < i1; ++i) {
       const int b = i & 1;
       double* pw = S.pw[b];
@@ -183,7 +192,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
         for (int h = 0; h < NH; ++h) {
           const int r = lane + 64 * h;
           v[h] = r > i + 1 ? x[h] * scal : (r == i + 1 ? 1.0 : 0.0);
-          if (r < NP) col[r] = v[h];
+          if (r < NP) col[tri_rx(r)] = v[h];
         }
         __builtin_amdgcn_wave_barrier();
         if (wave == 3) {
@@ -205,9 +214,9 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
       double vR[T], vC[T];
 #pragma unroll
-      for (int jj = 0; jj < T; ++jj) vC[jj] = col[c0 + jj];
+      for (int jj = 0; jj < T; ++jj) vC[jj] = col[C0 + jj];
 #pragma unroll
-      for (int ii = 0; ii < T; ++ii) vR[ii] = col[r0 + ii];
+      for (int ii = 0; ii < T; ++ii) vR[ii] = col[R0 + ii];
 #pragma unroll
       for (int ii = 0; ii < T; ++ii) {
         double ps = 0;
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
 #pragma unroll
           for (int jj = 0; jj < T; ++jj) ps = fma(a[ii][jj], vC[jj], ps);
         }
-        S.part[tc][r0 + ii] = ps;
+        S.part[tc][R0 + ii] = ps;
       }
       // column i+1 before this step's update, from the threads of its tile column
       if (tc == (i + 1 - base) / T) {
@@ -224,25 +233,25 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
         for (int jj = 0; jj < T; ++jj)
           if (jj == j0) {
 #pragma unroll
-            for (int ii = 0; ii < T; ++ii) pub[r0 + ii] = a[ii][jj];
+            for (int ii = 0; ii < T; ++ii) pub[R0 + ii] = a[ii][jj];
           }
       }
       lds_barrier();
       // (4) p = tau A v and the waves' shares of v^T p.  Rows above the block
       // (later stages) read stale sums and get p = 0 (they are <= i).
       if (wave >= PW0) {
-        const int row = t - (256 - NP);
+        const int row = t - (256 - NP), rowx = tri_rx(row);
         double vp = 0.0;
         if (row >= 0) {
           double s4[4];
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][row];
+          for (int qq = 0; qq < 4; ++qq) s4[qq] = S.part[qq][rowx];
 #pragma unroll
-          for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][row];
+          for (int qq = 4; qq < 16; ++qq) s4[qq & 3] += S.part[qq][rowx];
           const double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
           const double pr = row > i ? tau * s : 0.0;
-          pw[row] = pr;
-          vp = col[row] * pr;
+          pw[rowx] = pr;
+          vp = col[rowx] * pr;
         }
         vp = wave_sum_d(vp);
         if (lane == 0) S.red[b][wave] = vp;
@@ -257,9 +266,9 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       if (act) {
         double wR[T], wC[T];
 #pragma unroll
-        for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[r0 + ii]);
+        for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[R0 + ii]);
 #pragma unroll
-        for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[c0 + jj]);
+        for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[C0 + jj]);
 #pragma unroll
         for (int ii = 0; ii < T; ++ii)
 #pragma unroll
@@ -269,13 +278,13 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
       // column i+1 after the update (the same operations as on the tiles:
       // v_{i+1} = 1, w_{i+1} = p_{i+1} + K)
       {
-        const double w1 = fma(K, 1.0, pw[i + 1]);
+        const double w1 = fma(K, 1.0, pw[tri_rx(i + 1)]);
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
           const int r = lane + 64 * h;
           if (r < NP) {
-            const double w = fma(K, v[h], pw[r]);
-            const double nc = fma(-v[h], w1, fma(-w, 1.0, pub[r]));
+            const double w = fma(K, v[h], pw[tri_rx(r)]);
+            const double nc = fma(-v[h], w1, fma(-w, 1.0, pub[tri_rx(r)]));
             x[h] = r > i ? nc : 0.0;
           }
         }

@@ -54,6 +54,25 @@ __device__ __forceinline__ double dpp_d(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// st(e, ld(e)) for e = e0, e0 + step, ... < n with B loads in flight per
+// thread: in a plain loop each load is waited for before its store.
+template <int B, typename Ld, typename St>
+__device__ __forceinline__ void copy_batched(int e0, int n, int step, Ld ld, St st) {
+  for (int eb = e0; eb < n; eb += B * step) {
+    double v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int e = eb + u * step;
+      v[u] = e < n ? ld(e) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int e = eb + u * step;
+      if (e < n) st(e, v[u]);
+    }
+  }
+}
+
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);

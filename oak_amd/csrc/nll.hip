@@ -69,7 +69,8 @@ __global__ __launch_bounds__(256) void k_nll_adjoint(int64_t z0, int N,
   const double* V = vec + (size_t)zl * 4 * NP;
   const double* Yg = ymat + (size_t)zl * NP * NP;
   if (YLDS) {
-    for (int e = tid; e < NP * NP; e += 256) Ys[e] = Yg[e];
+    copy_batched<16>(tid, NP * NP, 256, [&](int e) { return Yg[e]; },
+                     [&](int e, double x) { Ys[e] = x; });
     __syncthreads();
   }
   const double* Y = YLDS ? Ys : Yg;

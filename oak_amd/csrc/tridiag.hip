@@ -624,10 +624,10 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
   const int zl = blockIdx.x * ZW + lane;
   const bool valid = zl < nzc && sel_ptr[z0 + zl + 1] > sel_ptr[z0 + zl];
   if (valid) {
-    for (int k = 0; k < NP; ++k) {
-      d[k][lane] = dvec[(size_t)k * nzc + zl];
-      e[k][lane] = evec[(size_t)k * nzc + zl];
-    }
+    copy_batched<16>(0, NP, 1, [&](int k) { return dvec[(size_t)k * nzc + zl]; },
+                     [&](int k, double x) { d[k][lane] = x; });
+    copy_batched<16>(0, NP, 1, [&](int k) { return evec[(size_t)k * nzc + zl]; },
+                     [&](int k, double x) { e[k][lane] = x; });
   }
   int st = 0;
   int64_t pos = 0;
@@ -754,7 +754,8 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
   if (stz & 3) return;  // failed zone: solved again by the fallback
   const int tid = threadIdx.x;
   double* Q = qmat + (size_t)zl * NP * NP;
-  for (int e = tid; e < NP * NP; e += BS) Z[(e / NP) * LD + e % NP] = Q[e];
+  copy_batched<16>(tid, NP * NP, BS, [&](int e) { return Q[e]; },
+                   [&](int e, double x) { Z[(e / NP) * LD + e % NP] = x; });
   const int64_t len = stz >> 2;
   const double2* lg = rlog + rlog_off(zl, cap);
   if (tid < NP && tid < len) buf[0][tid] = lg[rlog_rec(tid)];
@@ -867,13 +868,15 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   if (stz & 3) return;  // failed zone: solved again by the fallback
   double* Q = qmat + (size_t)zl * NP * NP;
   // upper half of the rows through LDS into registers, then the lower half
-  for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + H + (e & 63)];
+  copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + H + (e & 63)]; },
+                   [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
   __syncthreads();
   double zr[H];
 #pragma unroll
   for (int c = 0; c < H; ++c) zr[c] = Z[tid * LD + c];
   __syncthreads();
-  for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + (e & 63)];
+  copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + (e & 63)]; },
+                   [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
   const int64_t len = stz >> 2;
   const double2* lg = rlog + rlog_off(zl, cap);
   if (tid < len) S.buf[0][tid] = lg[rlog_rec(tid)];
@@ -1098,7 +1101,8 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   av[tid] = bv[tid] - ma;
   __syncthreads();
   colsum1(H);  // upper half still staged
-  for (int e = tid; e < NP * H; e += NP) Z[(e >> 6) * LD + (e & 63)] = Q[(e >> 6) * NP + (e & 63)];
+  copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + (e & 63)]; },
+                   [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
   __syncthreads();
   colsum1(0);
   double ux, unused;

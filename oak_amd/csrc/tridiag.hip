@@ -1091,10 +1091,12 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   // refinement a = a0 + U diag(1/mu) U^T (b - M a0); M symmetric: column tid
   const double* M = mat + (size_t)zl * NP * NP;
   double mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 2
-  for (int c0 = 0; c0 < NP; c0 += 8) {
+  for (int c0 = 0; c0 < NP; c0 += 64) {  // 64 loads in flight (the HBM latency once per half)
+    double mv[64];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) mq[u] = fma(M[(size_t)(c0 + u) * NP + tid], av[c0 + u], mq[u]);
+    for (int u = 0; u < 64; ++u) mv[u] = M[(size_t)(c0 + u) * NP + tid];
+#pragma unroll
+    for (int u = 0; u < 64; ++u) mq[u & 7] = fma(mv[u], av[c0 + u], mq[u & 7]);
   }
   const double ma = ((mq[0] + mq[1]) + (mq[2] + mq[3])) + ((mq[4] + mq[5]) + (mq[6] + mq[7]));
   __syncthreads();

@@ -177,7 +177,7 @@ def run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea, label):
     return ms, phase, st
 
 
-def roofline(F, phase, mixed, eig_name, config):
+def roofline(F, phase, mixed, eig_name, config, N):
     """Dominant phase of the step and its achieved rate against the MI355X peak
     of the arithmetic it runs on (FP32 MFMA for the mixed contractions, FP64
     otherwise)."""
@@ -187,9 +187,10 @@ def roofline(F, phase, mixed, eig_name, config):
     if fused:
         work["update"] += work["tmat"]
     kern = max(work, key=lambda k: phase[k])
+    gk = "k_gram128" if N > 64 else "k_gram"
     names = {
-        "gram": ("k_gram128<float> (S^T R^-1 S from FP32 S, v_mfma_f64_16x16x4f64)" if mixed else
-                 "k_gram128 (S^T R^-1 S, v_mfma_f64_16x16x4f64)"),
+        "gram": ("%s<float> (S^T R^-1 S from FP32 S, v_mfma_f64_16x16x4f64)" % gk if mixed else
+                 "%s (S^T R^-1 S, v_mfma_f64_16x16x4f64)" % gk),
         "eig": eig_name,
         "tmat": "k_tmat (T = Y Y^T Omega, v_mfma_f64_16x16x4f64)",
         "update": ("k_update_f32 (Ea = xa + A' W, v_mfma_f32_16x16x4f32)" if mixed else
@@ -382,7 +383,7 @@ def main():
                    "mean_p": float(counts.mean()), "median_p": float(np.median(counts)),
                    "max_p": int(counts.max()),
                    "phase_ms": {k: round(v, 3) for k, v in phase.items()}},
-        "roofline": roofline(F, phase, head == "mixed", eig_name, args.config),
+        "roofline": roofline(F, phase, head == "mixed", eig_name, args.config, N),
     }
     if "mixed" in res and "fp64" in res:
         ms64, ph64, _ = res["fp64"]

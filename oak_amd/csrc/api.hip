@@ -26,6 +26,7 @@ struct oakg_ctx {
   std::vector<int64_t> counts;
   bool timing = false;
   oakg::PhaseTimer timer;
+  size_t ws_budget = (size_t)8 << 30;  // chunk workspace of one zone loop (all slots)
   double jacobi_tol = 0;  // 0 = default for the precision
   int max_sweeps = 40;
   int eig_method = 1;     // 1 = tridiagonal QL (default), 0 = one-sided Jacobi
@@ -321,7 +322,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const bool ql = c->eig_method == 1;
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
-  const size_t budget = (size_t)8 << 30;
+  const size_t budget = c->ws_budget;
   const int nslot = (!c->timing && nz >= 4096) ? OAKG_PIPE_SLOTS : 1;
   int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
   zc = std::min<int64_t>(zc, 1 << 20);
@@ -812,6 +813,13 @@ int oakg_create(oakg_ctx** out, int device, int precision) {
   if (!c) return OAKG_ENOMEM;
   c->device = device;
   c->precision = precision;
+  // chunk workspace: a quarter of the device memory, between 8 and 64 GiB
+  // (larger chunks leave fewer part-filled rounds of workgroups at the end
+  // of each kernel; 64 GiB holds config 2's 64 800 zones of N = 128)
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    c->ws_budget = std::min<size_t>(std::max<size_t>(prop.totalGlobalMem / 4, (size_t)8 << 30),
+                                    (size_t)64 << 30);
   *out = c;
   return OAKG_OK;
 }

@@ -15,6 +15,10 @@
 #include "common.hpp"
 #include "internal.hpp"
 
+#ifndef OAKG_PIPE_SLOTS
+#define OAKG_PIPE_SLOTS 3  // zone chunks in flight on as many streams
+#endif
+
 struct oakg_ctx {
   int device = 0;
   int precision = OAKG_FP64;
@@ -35,14 +39,10 @@ struct oakg_ctx {
   const int64_t* last_sel_ptr = nullptr;
   int64_t* d_stats = nullptr;  // device: error word + reduced statistics
   bool counts_ready = true;
-  // second stream for the chunk pipeline (default priority) and its events
-  hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  // internal streams of the chunk pipeline (default priority) and their events
+  hipStream_t aux[OAKG_PIPE_SLOTS - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[OAKG_PIPE_SLOTS - 1] = {};
 };
-
-#ifndef OAKG_PIPE_SLOTS
-#define OAKG_PIPE_SLOTS 3  // zone chunks in flight on as many streams
-#endif
 
 namespace oakg {
 
@@ -345,7 +345,8 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
     fill_i32(sweeps_all, nz, 0, c->stream);
   }
-  hipStream_t S[3] = {c->stream, c->stream, c->stream};
+  hipStream_t S[OAKG_PIPE_SLOTS];
+  for (int i = 0; i < OAKG_PIPE_SLOTS; ++i) S[i] = c->stream;
   if (nslot > 1) {
     if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
     check(c, hipEventRecord(c->ev_fork, S[0]), "event");
@@ -828,7 +829,7 @@ int oakg_destroy(oakg_ctx* c) {
   if (!c) return OAKG_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < OAKG_PIPE_SLOTS - 1; ++i) {
     if (c->aux[i]) {
       (void)hipStreamSynchronize(c->aux[i]);
       (void)hipStreamDestroy(c->aux[i]);

@@ -20,8 +20,8 @@
 //                  Y = U diag(mu^-1/4), with mu = max(lambda, 1)
 //                  (rrsqrt.F90:137-142, 176-177); mu itself is kept beside a
 //                  and v for the likelihood.
-//   k_tql_apply128 (N > 64) the rotations on rows held half in registers,
-//                  half in LDS, then the epilogue of k_eigvec_epi.
+//   k_tql_apply128 (N > 64) the rotations on rows held in registers, then
+//                  the epilogue of k_eigvec_epi.
 #include <type_traits>
 
 #include "analysis.hpp"
@@ -826,21 +826,23 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
 }
 
 // ---------------------------------------------------------------------------
-// U = Q Z for NP = 128 with half of each row in registers.  Thread r holds
-// columns 64..127 of row r in VGPRs and columns 0..63 in LDS (66 KB, so two
-// zones share a CU).  The chains run from their first position m-1 down to l,
-// so most rotations fall on the register half: there the chain runs as
-// unrolled, branch-free segments of 16 positions, each segment it meets with
-// the identity at the positions it does not cover (a jump into the middle of
-// an unrolled sequence is not structured control flow, and the compiler's
-// rewriting of it copies the register half around).  Position 63 couples the halves; the
-// LDS part is the loop of k_tql_apply.  The arithmetic of every rotation is
-// that of k_tql_apply.
+// U = Q Z for NP = 128 with the rows in registers.  Thread r holds columns
+// 64..127 of row r in zr and 0..63 in zw (VGPRs, with part of zw in AGPRs:
+// one wave per SIMD, two zones per CU).  A chain (one QL sweep, from its
+// first position m-1 down to l) runs as unrolled, branch-free segments of 16
+// positions, each segment it meets with the identity at the positions it
+// does not cover (a jump into the middle of an unrolled sequence is not
+// structured control flow, and the compiler's rewriting of it copies the
+// registers around); the (c, s) of a segment come from a per-wave,
+// position-indexed table in LDS.  Position 63 couples the two halves.  The
+// arithmetic of every rotation is that of k_tql_apply.  The LDS (66 KB) is
+// used by the epilogue, to stage a half of U for its column sums.
 // ---------------------------------------------------------------------------
 struct Apply128Lds {
-  double z[8 + 128 * 65];  // columns 0..63, row stride 65, 8 doubles of room in front
+  double z[8 + 128 * 65];  // a staged half of U, row stride 65 (8 doubles of room in front)
   double2 buf[2][128 + 12];
   double2 pr[2][64];       // per wave: (c, s) of positions 64..127 for the current chain
+  double2 prl[2][64];      // per wave: (c, s) of positions 0..62 for the current chain
   double ev[5][128];       // epilogue vectors
 };
 
@@ -877,6 +879,10 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   __syncthreads();
   copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + (e & 63)]; },
                    [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
+  __syncthreads();
+  double zw[H];  // columns 0..63 (the compiler keeps part of them in AGPRs)
+#pragma unroll
+  for (int c = 0; c < H; ++c) zw[c] = Z[tid * LD + c];
   const int64_t len = stz >> 2;
   const double2* lg = rlog + rlog_off(zl, cap);
   if (tid < len) S.buf[0][tid] = lg[rlog_rec(tid)];
@@ -933,62 +939,39 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
     }
     if (l < H) {
       // position 63 (if the chain reaches it) couples zr[0] = column 64 with
-      // column 63 in LDS; then positions 62 .. l in LDS as in k_tql_apply
-      double* p;
-      double carry;
-      int n_lo;
-      lds_d* cd;
+      // zw[63]; then positions 62 .. l as unrolled segments on zw, like the
+      // upper half (table prl of positions 0..62)
+      const int lane = tid & 63;
+      double2* prl = S.prl[tid >> 6];
+      prl[lane] = (lane >= l && lane <= m - 1 && lane <= H - 2) ? S.buf[b][1 + (m - 1 - lane)]
+                                                                 : make_double2(1.0, 0.0);
       if (m - 1 >= H - 1) {
         const double2 q = S.buf[b][1 + (m - 1) - (H - 1)];
-        double* z63 = Z + tid * LD + (H - 1);
-        const double hh = zr[0], zj = *z63;
+        const double hh = zr[0], zj = zw[H - 1];
         zr[0] = fma(q.y, zj, q.x * hh);
-        carry = fma(q.x, zj, -q.y * hh);
-        p = z63 - 1;
-        n_lo = (H - 1) - l;
-        cd = (lds_d*)(S.buf[b] + 1 + (m - 1) - (H - 2));
-      } else {
-        p = Z + tid * LD + (m - 1);
-        carry = p[1];
-        n_lo = cnt;
-        cd = (lds_d*)(S.buf[b] + 1);
+        zw[H - 1] = fma(q.x, zj, -q.y * hh);
       }
-      asm volatile("" : "+v"(cd));
-      int k = 0;
-#define CS(J) make_double2(cd[2 * (J)], cd[2 * (J) + 1])
-#define OAK_ROT(Q, ZV, DST)                  \
-  DST = fma(Q.y, ZV, Q.x * carry);           \
-  carry = fma(Q.x, ZV, -Q.y * carry);
-      if (n_lo >= 8) {
-        double a0 = p[0], a1 = p[-1], a2 = p[-2], a3 = p[-3];
-        double2 qa0 = CS(0), qa1 = CS(1), qa2 = CS(2), qa3 = CS(3);
-        for (; k + 8 <= n_lo; k += 8) {
-          const double b0 = p[-4], b1 = p[-5], b2 = p[-6], b3 = p[-7];
-          const double2 qb0 = CS(4), qb1 = CS(5), qb2 = CS(6), qb3 = CS(7);
-          OAK_ROT(qa0, a0, p[1])
-          OAK_ROT(qa1, a1, p[0])
-          OAK_ROT(qa2, a2, p[-1])
-          OAK_ROT(qa3, a3, p[-2])
-          a0 = p[-8]; a1 = p[-9]; a2 = p[-10]; a3 = p[-11];
-          qa0 = CS(8); qa1 = CS(9); qa2 = CS(10); qa3 = CS(11);
-          OAK_ROT(qb0, b0, p[-3])
-          OAK_ROT(qb1, b1, p[-4])
-          OAK_ROT(qb2, b2, p[-5])
-          OAK_ROT(qb3, b3, p[-6])
-          p -= 8;
-          cd += 16;
+      auto segment_lo = [&](auto top) {
+        constexpr int T = decltype(top)::value;
+        if (m - 1 < T - 15 || l > T) return;
+        double2 qv[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+          if (T - t >= 0) qv[t] = prl[T - t];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int J = T - t;
+          if (J >= 0) {
+            const double hh = zw[J + 1], zj = zw[J];
+            zw[J + 1] = fma(qv[t].y, zj, qv[t].x * hh);
+            zw[J] = fma(qv[t].x, zj, -qv[t].y * hh);
+          }
         }
-      }
-      for (; k < n_lo; ++k) {
-        const double zv = p[0];
-        const double2 q = CS(0);
-        OAK_ROT(q, zv, p[1])
-        p -= 1;
-        cd += 2;
-      }
-#undef OAK_ROT
-#undef CS
-      p[1] = carry;  // column l
+      };
+      segment_lo(std::integral_constant<int, 62>{});
+      segment_lo(std::integral_constant<int, 46>{});
+      segment_lo(std::integral_constant<int, 30>{});
+      segment_lo(std::integral_constant<int, 14>{});
     }
     if (pf) S.buf[b ^ 1][tid] = pv;
     __syncthreads();
@@ -996,11 +979,9 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
     b ^= 1;
   }
   // ---- epilogue: the quantities of k_eigvec_epi from the rows in hand ----
-  // Thread r holds the upper half of row r of U in registers; the lower half
-  // is in LDS and, while the upper half is staged there, in Q (the output
-  // location, L2-resident).  Column sums U^T w are taken from the half staged
-  // in LDS (column cc, rows 64 hf .. 64 hf + 63 per thread); row products U x
-  // from the registers and the lower half.
+  // Thread r holds row r of U in registers (zw, zr).  Column sums U^T w are
+  // taken from a half staged in LDS (column cc, rows 64 hf .. 64 hf + 63 per
+  // thread); row products U x from the registers.
   double* mu = S.ev[0];  // mu_j = max(lambda_j, 1) of column j
   double* bv = S.ev[1];  // b
   double* xv = S.ev[2];  // U^T b / mu, then U^T (b - M a0) / mu
@@ -1011,6 +992,8 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   mu[tid] = fmax(dvec[(size_t)tid * nzc + zl], 1.0);
   bv[tid] = V[tid];
   const int cc = tid & 63, hf = tid >> 6;
+#pragma unroll
+  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zw[c];  // the lower half staged
   __syncthreads();
   // U^T b and U^T 1_N over the staged half (column offset co)
   auto colsum2 = [&](int co) {
@@ -1051,7 +1034,7 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   };
   // (U x)_r and (U y)_r in one pass: the lower half of row r from lo (LDS row
   // or global row), the upper half from the registers; 16 entries at a time
-  auto rowdot2 = [&](const double* lo, const double* x, const double* y, double& tx, double& ty) {
+  auto rowdot2 = [&](const double* x, const double* y, double& tx, double& ty) {
     tx = 0;
     ty = 0;
 #pragma unroll
@@ -1060,7 +1043,7 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
       double2 uu[8], xx[8], yy[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        if (c0 < H) uu[u] = reinterpret_cast<const double2*>(lo + c0)[u];
+        if (c0 < H) uu[u] = make_double2(zw[c0 + 2 * u], zw[c0 + 2 * u + 1]);
         else uu[u] = make_double2(zr[c0 - H + 2 * u], zr[c0 - H + 2 * u + 1]);
         xx[u] = reinterpret_cast<const double2*>(x + c0)[u];
         if (y) yy[u] = reinterpret_cast<const double2*>(y + c0)[u];
@@ -1077,15 +1060,12 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
     }
   };
   colsum2(0);  // the lower half is in LDS
-  for (int e = tid; e < NP * H; e += NP) Q[(e >> 6) * NP + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
-  __syncthreads();
 #pragma unroll
   for (int c = 0; c < H; ++c) Z[tid * LD + c] = zr[c];
-  __threadfence_block();  // the lower half in Q is read back by its own row's thread
   __syncthreads();
   colsum2(H);  // upper half
   double a0, vr;
-  rowdot2(Q + (size_t)tid * NP, xv, yv, a0, vr);
+  rowdot2(xv, yv, a0, vr);
   av[tid] = a0;
   __syncthreads();
   // refinement a = a0 + U diag(1/mu) U^T (b - M a0); M symmetric: column tid
@@ -1103,12 +1083,12 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   av[tid] = bv[tid] - ma;
   __syncthreads();
   colsum1(H);  // upper half still staged
-  copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + (e & 63)]; },
-                   [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
+#pragma unroll
+  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zw[c];
   __syncthreads();
   colsum1(0);
   double ux, unused;
-  rowdot2(Z + tid * LD, xv, nullptr, ux, unused);
+  rowdot2(xv, nullptr, ux, unused);
   const double ar = a0 + ux;
   av[tid] = vr * vr;
   yv[tid] = 1.0 / sqrt(sqrt(mu[tid]));

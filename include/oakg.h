@@ -115,8 +115,9 @@ typedef struct {
 
 int oakg_version(void);
 /* The context keeps a device workspace for the zone chunks of a call, sized
-   to a quarter of the device memory between 8 and 64 GiB (allocated on first
-   use and kept until oakg_destroy). */
+   to a quarter of the device memory between 8 and 64 GiB, or to
+   OAKG_WORKSPACE_BYTES from the environment (at least 1 MiB) when that is set
+   (allocated on first use and kept until oakg_destroy). */
 int oakg_create(oakg_ctx** ctx, int device, int precision);
 int oakg_destroy(oakg_ctx* ctx);
 const char* oakg_last_error(const oakg_ctx* ctx);

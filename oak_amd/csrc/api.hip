@@ -3,6 +3,7 @@
 // the error contract (outputs untouched on error with host pointers).
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -817,10 +818,15 @@ int oakg_create(oakg_ctx** out, int device, int precision) {
   // chunk workspace: a quarter of the device memory, between 8 and 64 GiB
   // (larger chunks leave fewer part-filled rounds of workgroups at the end
   // of each kernel; 64 GiB holds config 2's 64 800 zones of N = 128)
+  // (OAKG_WORKSPACE_BYTES in the environment sets it instead, at least 1 MiB)
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess)
     c->ws_budget = std::min<size_t>(std::max<size_t>(prop.totalGlobalMem / 4, (size_t)8 << 30),
                                     (size_t)64 << 30);
+  if (const char* s = getenv("OAKG_WORKSPACE_BYTES")) {
+    const long long v = atoll(s);
+    if (v > 0) c->ws_budget = std::max<size_t>((size_t)v, (size_t)1 << 20);
+  }
   *out = c;
   return OAKG_OK;
 }

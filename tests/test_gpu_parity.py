@@ -1,5 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle and with
 OAK's known answers.  Needs an MI355X."""
+import os
+
 import numpy as np
 import pytest
 
@@ -266,8 +268,9 @@ def test_ensemble_deep_columns_vs_oracle(oak, N):
 @pytest.mark.parametrize("N", [16, 128])
 def test_chunk_pipeline_matches_in_order(oak, N):
     """Without phase timing the zone chunks run on several streams; with it
-    they run in order on one stream.  Every zone is computed alone, so the
-    two schedules (and their different chunk sizes) give the same bits."""
+    they run in order on one stream; with a small workspace each stream runs
+    many small chunks in turn.  Every zone is computed alone, so the three
+    schedules (and their different chunk sizes) give the same bits."""
     from oak_amd import synth
     from oak_amd._lib import Context
     _, A = oak
@@ -276,12 +279,19 @@ def test_chunk_pipeline_matches_in_order(oak, N):
     assert P.nzones >= 4096  # the pipeline runs from 4096 zones on
     H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
     out = []
-    for timing in (False, True):
-        c = Context(0)
+    for timing, ws in ((False, None), (True, None), (False, 16 << 20)):
+        # a 16 MiB workspace: chunks of 64-400 zones, each stream running many
+        if ws:
+            os.environ["OAKG_WORKSPACE_BYTES"] = str(ws)
+        try:
+            c = Context(0)
+        finally:
+            os.environ.pop("OAKG_WORKSPACE_BYTES", None)
         c.set_timing(timing)
         out.append(A.ensemble_analysis(_geometry(P), H, P.Hshift, P.yo, P.rmse2, P.E, ctx=c))
         c.close()
     np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
     assert not np.array_equal(out[0], P.E)
 
 

@@ -669,25 +669,23 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
       // wait for the loads above here: left to the compiler, the wait lands
       // in the loop header and runs on every step
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-      for (; i >= l; --i) {
+      // Branch-free step: h = 0 exactly (the block splits at i + 1) means
+      // f = g = 0, so the same formulas with 1/r taken as 1 give
+      // e_{i+1} = 0 and d_{i+1} = d_{i+1} - p; the step then logs nothing
+      // and ends the sweep.
+      for (; i >= l && !under; --i) {
         // operands of the next step, loaded ahead (unconditionally: a load
         // under a branch is waited for at the join; unused when i == l)
         const int im = i > 0 ? i - 1 : 0;
         const double ein = e[im][lane], din = d[im][lane];
         const double f = s * ei, bb = c * ei;
         const double h = fma(f, f, g * g);
-        if (h == 0.0) {  // exact zero: the block splits at i + 1
-          d[i + 1][lane] = dip1 - p;
-          e[i + 1][lane] = 0.0;
-          e[m][lane] = 0.0;
-          under = true;
-          break;
-        }
+        under = h == 0.0;
         // r = sqrt(h), s' = f / r, c' = g / r; with g2 = d_{i+1} - p,
         // (d_i - g2) s' + 2 c' bb = X / r, X formed while 1/r is computed
         const double g2 = dip1 - p;
         const double X = fma(di - g2, f, 2.0 * g * bb);
-        const double ri = rsqrt_acc(h);
+        const double ri = rsqrt_acc(under ? 1.0 : h);
         const double r = h * ri;
         e[i + 1][lane] = r;
         s = f * ri;
@@ -698,16 +696,18 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
         d[i + 1][lane] = dn;
         g = fma(c, rr, -bb);
         lg[rlog_rec(pos)] = make_double2(c, s);
-        ++pos;
-        if (i + 1 < m && fabs(r) <= eps * (fabs(dn) + fabs(dnext2))) jmin = i + 1;
+        pos += under ? 0 : 1;
+        const bool small = fabs(r) <= eps * (fabs(dn) + fabs(dnext2));
+        jmin = (!under & (i + 1 < m) & small) ? i + 1 : jmin;  // no short circuit: no branch
         dnext2 = dn;
         ei = ein;
         dip1 = di;
         di = din;
       }
       lg[rlog_rec(hdr)] = make_double2((double)l + 65536.0 * (double)m, (double)(pos - hdr - 1));
-      if (under) {
-        m = i + 1;
+      if (under) {  // the loop stepped once past the split at i + 1
+        e[m][lane] = 0.0;
+        m = i + 2;
         continue;
       }
       const double dl2 = dl - p;

@@ -677,34 +677,35 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
       double xar = x0 + dxp;
       if (xar - maxcorr > x0) xar = x0 + maxcorr;
       if (xar < x0 - maxcorr) xar = x0 - maxcorr;
-      f4 acc[8];
+      // two passes over the output tiles (4 each) keep the accumulators small
+#pragma unroll 1
+      for (int h = 0; h < 2; ++h) {
+        const double* xh = xr + 64 * h * LDT;
+        f4 acc[4];
 #pragma unroll
-      for (int J = 0; J < 8; ++J) acc[J] = f4{0, 0, 0, 0};
+        for (int J = 0; J < 4; ++J) acc[J] = f4{0, 0, 0, 0};
 #pragma unroll
-      for (int p = 0; p < P128 / 8; ++p) {
-#pragma unroll
-        for (int H = 0; H < 8; H += 4) {  // four tiles at a time
+        for (int p = 0; p < P128 / 8; ++p) {
           asm volatile("" ::: "memory");
           double2 A[4];
 #pragma unroll
-          for (int J = 0; J < 4; ++J)
-            A[J] = *reinterpret_cast<const double2*>(xr + 16 * (H + J) * LDT + 8 * p);
+          for (int J = 0; J < 4; ++J) A[J] = *reinterpret_cast<const double2*>(xh + 16 * J * LDT + 8 * p);
 #pragma unroll
           for (int J = 0; J < 4; ++J)
-            acc[H + J] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].x, bf[2 * p], acc[H + J], 0, 0, 0);
+            acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].x, bf[2 * p], acc[J], 0, 0, 0);
 #pragma unroll
           for (int J = 0; J < 4; ++J)
-            acc[H + J] =
-                __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].y, bf[2 * p + 1], acc[H + J], 0, 0, 0);
+            acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].y, bf[2 * p + 1], acc[J], 0, 0, 0);
         }
-      }
-      if (rok) {
+        if (rok) {
 #pragma unroll
-        for (int J = 0; J < 8; ++J) {
+          for (int J = 0; J < 4; ++J) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int col = 16 * J + 4 * kk + r;
-            if (col < N) Out[(i0 + row) + (int64_t)col * ldO] = (float)(xar + sc_infl * (double)acc[J][r]);
+            for (int r = 0; r < 4; ++r) {
+              const int col = 64 * h + 16 * J + 4 * kk + r;
+              if (col < N)
+                Out[(i0 + row) + (int64_t)col * ldO] = (float)(xar + sc_infl * (double)acc[J][r]);
+            }
           }
         }
       }

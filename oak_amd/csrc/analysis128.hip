@@ -778,6 +778,231 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same for FP32 members (mixed precision): Y is staged in LDS rounded to
+// FP32, Y Y^T runs on v_mfma_f32_16x16x4f32 and T Omega is kept in FP32, the
+// operand precision of the FP32 update that consumes it (the Omega row sums
+// and xa = x0 + Sf a stay in FP64).  The buffers take 78 KB, so two zones
+// share a CU; four waves per zone: wave w owns tile rows w and 7 - w.
+//   FP32 MFMA layout: A (16x4) lane l holds A[l % 16][l / 16], B (4x16)
+//   B[l / 16][l % 16], D (16x16) register r holds D[4 (l / 16) + r][l % 16].
+// ---------------------------------------------------------------------------
+constexpr int LDF = P128 + 4;
+
+struct XformLdsF {
+  float t[P128 * LDF];
+  double u[2][P128];   // half sums of T v
+  double w[2][P128];   // half sums of T over the first N-1 entries
+  double v[P128];
+  double a[P128 + 2];  // a / scaling, then v.w~
+};
+
+__global__ __launch_bounds__(256) void k_xform128f(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+                                                   const int64_t* __restrict__ zone_ptr,
+                                                   const double* __restrict__ ymat,
+                                                   const double* __restrict__ vec,
+                                                   const double* __restrict__ X0,
+                                                   const float* __restrict__ S, int64_t ldS,
+                                                   float* __restrict__ Out, int64_t ldO,
+                                                   double scaling, double infl, double maxcorr) {
+#pragma clang fp contract(off)
+  __shared__ XformLdsF L;
+  const int zl = blockIdx.x;
+  const int64_t z = z0 + zl;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, kk = lane >> 4;
+  const int64_t i0 = zone_ptr[z];
+  const int64_t sz = zone_ptr[z + 1] - i0;
+  // k permuted within pairs of MFMA steps (as in k_xform128): in steps 2p and
+  // 2p+1 lane group kk covers k = 8p + 2kk and 8p + 2kk + 1
+  float b[P128 / 4];
+  double x0 = 0.0;
+  auto load_strip = [&](int64_t rb) {
+    const int64_t row = rb + li;
+    const int64_t rr = i0 + (row < sz ? row : 0);
+    x0 = X0[rr];
+#pragma unroll
+    for (int q = 0; q < P128 / 4; ++q) {
+      const int k = 8 * (q >> 1) + 2 * kk + (q & 1);
+      b[q] = S[rr + (int64_t)(k < N ? k : 0) * ldS];
+    }
+  };
+  const double* Vz = vec + (size_t)zl * 4 * P128;
+  if (tid < P128) L.a[tid] = Vz[P128 + tid] / scaling;
+  else L.v[tid - P128] = tid - P128 < N ? Vz[2 * P128 + tid - P128] : 0.0;
+  if (sel_ptr[z + 1] == sel_ptr[z]) {
+    // no observations: T = I (a = 0), the update copies the forecast anomalies
+    for (int e = tid; e < P128 * P128; e += 256)
+      L.t[(e >> 7) * LDF + (e & 127)] = (e >> 7) == (e & 127) ? 1.0f : 0.0f;
+    if (sz > 0) load_strip(16 * wave);
+    lds_barrier();
+  } else {
+    const double2* Y2 = reinterpret_cast<const double2*>(ymat + (size_t)zl * P128 * P128);
+#pragma unroll
+    for (int it0 = 0; it0 < P128 * P128 / 512; it0 += 16) {
+      double2 yv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) yv[u] = Y2[tid + 256 * (it0 + u)];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int e = tid + 256 * (it0 + u);
+        *reinterpret_cast<float2*>(L.t + (e >> 6) * LDF + 2 * (e & 63)) =
+            make_float2((float)yv[u].x, (float)yv[u].y);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (sz > 0) load_strip(16 * wave);
+    lds_barrier();
+    {
+      const int R0 = wave, R1 = 7 - wave;  // tiles (R0, 0..R0) and (R1, 0..R1)
+      f4 acc0[4], acc1[8];
+#pragma unroll
+      for (int J = 0; J < 4; ++J) acc0[J] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int J = 0; J < 8; ++J) acc1[J] = f4{0, 0, 0, 0};
+      const float* yr = L.t + li * LDF + 2 * kk;
+#pragma unroll 1
+      for (int p = 0; p < P128 / 8; ++p) {
+        const float2 y0 = *reinterpret_cast<const float2*>(yr + 16 * R0 * LDF + 8 * p);
+        const float2 y1 = *reinterpret_cast<const float2*>(yr + 16 * R1 * LDF + 8 * p);
+        float2 yb[8];
+#pragma unroll
+        for (int J = 0; J < 8; ++J) yb[J] = *reinterpret_cast<const float2*>(yr + 16 * J * LDF + 8 * p);
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+          if (J <= R0) {
+            acc0[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(y0.x, yb[J].x, acc0[J], 0, 0, 0);
+            acc0[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(y0.y, yb[J].y, acc0[J], 0, 0, 0);
+          }
+#pragma unroll
+        for (int J = 0; J < 8; ++J)
+          if (J <= R1) {
+            acc1[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(y1.x, yb[J].x, acc1[J], 0, 0, 0);
+            acc1[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(y1.y, yb[J].y, acc1[J], 0, 0, 0);
+          }
+      }
+      lds_barrier();  // every wave has read Y
+      // D[16 I + 4 (lane / 16) + r][16 J + lane % 16]; mirrored entries keep T symmetric
+      auto put = [&](int I, int J, const f4& acc) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = 16 * I + 4 * kk + r, n = 16 * J + li;
+          L.t[m * LDF + n] = acc[r];
+          L.t[n * LDF + m] = acc[r];
+        }
+      };
+#pragma unroll
+      for (int J = 0; J < 4; ++J)
+        if (J <= R0) put(R0, J, acc0[J]);
+#pragma unroll
+      for (int J = 0; J < 8; ++J)
+        if (J <= R1) put(R1, J, acc1[J]);
+    }
+    lds_barrier();
+    // Omega on the right of the symmetric T, as in k_xform128 (row sums read
+    // as column sums, accumulated in FP64)
+    const double wN = 1.0 / sqrt((double)N);
+    const double vN = L.v[N - 1];
+    const double sgv = vN >= 0 ? 1.0 : -1.0;
+    const double alv = -1.0 / (fabs(vN) + 1.0);
+    const double alw = -1.0 / (wN + 1.0);
+    {
+      const int r = tid & 127, h = tid >> 7;  // half h of the column
+      double su = 0, sw = 0;
+#pragma unroll 8
+      for (int q = 64 * h; q < 64 * h + 64; ++q) {
+        const double tq = L.t[q * LDF + r];
+        su = fma(tq, L.v[q], su);
+        sw += q < N - 1 ? tq : 0.0;
+      }
+      L.u[h][r] = su;
+      L.w[h][r] = sw;
+    }
+    if (wave == 0) {
+      double s = 0;
+      for (int q = lane; q < N - 1; q += 64) s = fma(L.v[q], wN, s);
+      s = wave_sum_fast(s);
+      if (lane == 0) L.a[P128] = s;
+    }
+    lds_barrier();
+    const int r = tid & 127;
+    const double u0 = L.t[(N - 1) * LDF + r];
+    const double u1 = L.u[0][r] + L.u[1][r];
+    const double u2 = wN * (L.w[0][r] + L.w[1][r]);
+    const double u3 = u1 + sgv * u0;
+    const double vw = L.a[P128];
+    const double c1 = u1 * wN;
+    const double c2 = alw * u2 + alv * alw * vw * u3;
+    const double c3 = alv * u3;
+    // X[s][r] = (T Omega)[r][s], in place (T symmetric: T[r][s] = t[s][r])
+#pragma unroll 4
+    for (int s = tid >> 7; s < N; s += 2) {
+      const double zw = s == N - 1 ? wN + 1.0 : wN;
+      const double vt = s < N - 1 ? L.v[s] : 0.0;
+      double x = L.t[s * LDF + r];
+      if (s == N - 1) x -= u0;
+      x += c1 + c2 * zw + c3 * vt;
+      L.t[s * LDF + r] = (float)x;
+    }
+    lds_barrier();
+  }
+  // out = Sf_z T: Out^T strip = T^T Sf_strip^T (as the FP32 branch of k_xform128)
+  const float* xr = L.t + li * LDF + 2 * kk;
+  const double sc_infl = scaling * infl;
+  const float inv = (float)(1.0 / scaling);
+  for (int64_t rb = 16 * wave; rb < sz; rb += 64) {
+    if (rb != 16 * wave) load_strip(rb);
+    const int64_t row = rb + li;
+    const bool rok = row < sz;
+    const float xb = (float)x0;
+    double dxp = 0.0;
+#pragma unroll
+    for (int q = 0; q < P128 / 4; ++q) {
+      const int kc = 8 * (q >> 1) + 2 * kk + (q & 1);
+      const bool ok = rok && kc < N;
+      const float ev = b[q];
+      if (ok) dxp = fma((double)ev - x0, L.a[kc], dxp);
+      b[q] = ok ? (ev - xb) * inv : 0.0f;
+    }
+    dxp += shfl_xor_d(dxp, 16);
+    dxp += shfl_xor_d(dxp, 32);
+    double xar = x0 + dxp;
+    if (xar - maxcorr > x0) xar = x0 + maxcorr;
+    if (xar < x0 - maxcorr) xar = x0 - maxcorr;
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const float* xh = xr + 64 * h * LDF;
+      f4 acc[4];
+#pragma unroll
+      for (int J = 0; J < 4; ++J) acc[J] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int p = 0; p < P128 / 8; ++p) {
+        asm volatile("" ::: "memory");
+        float2 A[4];
+#pragma unroll
+        for (int J = 0; J < 4; ++J) A[J] = *reinterpret_cast<const float2*>(xh + 16 * J * LDF + 8 * p);
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+          acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[J].x, b[2 * p], acc[J], 0, 0, 0);
+#pragma unroll
+        for (int J = 0; J < 4; ++J)
+          acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[J].y, b[2 * p + 1], acc[J], 0, 0, 0);
+      }
+      if (rok) {
+#pragma unroll
+        for (int J = 0; J < 4; ++J) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int col = 64 * h + 16 * J + 4 * kk + r;
+            if (col < N)
+              Out[(i0 + row) + (int64_t)col * ldO] = (float)(xar + sc_infl * (double)acc[J][r]);
+          }
+        }
+      }
+    }
+  }
+}
+
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_GRAM);
   if (A.prec == 1)
@@ -806,9 +1031,9 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   // transform and update in one kernel (the timer books it as the update)
   if (tm) tm->mark(PH_UPDATE);
   if (A.prec == 1)
-    hipLaunchKernelGGL((k_xform128<1, float>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
-                       A.zone_ptr, A.ymat, A.vec, A.X0, A.S32, A.ldS, A.xa, A.Out32, A.ldO,
-                       A.scaling, A.infl, A.maxcorr);
+    hipLaunchKernelGGL(k_xform128f, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr,
+                       A.zone_ptr, A.ymat, A.vec, A.X0, A.S32, A.ldS, A.Out32, A.ldO, A.scaling,
+                       A.infl, A.maxcorr);
   else if (A.mode == 0)
     hipLaunchKernelGGL((k_xform128<0, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,

@@ -197,10 +197,11 @@ def roofline(F, phase, mixed, eig_name, config, N):
                    "k_update (Ea = xa + A' W, v_mfma_f64_16x16x4f64)"),
     }
     if fused:
-        names["update"] = ("k_xform128 (T = Y Y^T Omega in LDS on v_mfma_f64_16x16x4f64, then "
-                           "Ea = xa + A' T on %s)" % ("v_mfma_f32_16x16x4f32" if mixed else
-                                                     "v_mfma_f64_16x16x4f64"))
-    peak = PEAK_FP32_MFMA if (mixed and kern == "update" and not fused) else PEAK_FP64
+        names["update"] = ("k_xform128f (T = Y Y^T Omega in LDS on v_mfma_f32_16x16x4f32, then "
+                           "Ea = xa + A' T on v_mfma_f32_16x16x4f32)" if mixed else
+                           "k_xform128 (T = Y Y^T Omega in LDS on v_mfma_f64_16x16x4f64, then "
+                           "Ea = xa + A' T on v_mfma_f64_16x16x4f64)")
+    peak = PEAK_FP32_MFMA if (mixed and kern == "update") else PEAK_FP64
     ach = work[kern] / (phase[kern] * 1e-3)
     return {"bound": "mfma", "kernel": names[kern], "achieved": ach / 1e12, "peak": peak / 1e12,
             "unit": "TFLOP/s", "frac": ach / peak, "traffic": phase_traffic(kern, mixed, config)}

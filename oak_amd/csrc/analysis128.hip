@@ -488,11 +488,10 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
                                                   int64_t ldO, double scaling, double infl,
                                                   double maxcorr) {
 #pragma clang fp contract(off)
-  // TE = double: FP64 state, update on FP64 MFMA (MODE 0: locAnalysis with
-  // Sf given, MODE 1: ensemble).  TE = float: FP32 members (MODE 1 only),
-  // update on FP32 MFMA with X rounded to FP32, Sf a and the mean in FP64.
+  // FP64 state, update on FP64 MFMA (MODE 0: locAnalysis with Sf given,
+  // MODE 1: ensemble); FP32 members run in k_xform128f.
   constexpr bool F32 = std::is_same<TE, float>::value;
-  static_assert(!F32 || MODE == 1, "FP32 state is the ensemble mode");
+  static_assert(!F32, "FP32 state runs in k_xform128f");
   __shared__ XformLds L;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
@@ -519,7 +518,7 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
     }
   };
   const double* Vz = vec + (size_t)zl * 4 * P128;
-  if (tid < P128) L.a[tid] = F32 ? Vz[P128 + tid] / scaling : Vz[P128 + tid];
+  if (tid < P128) L.a[tid] = Vz[P128 + tid];
   else if (tid < 2 * P128) L.v[tid - P128] = tid - P128 < N ? Vz[2 * P128 + tid - P128] : 0.0;
   if (sel_ptr[z + 1] == sel_ptr[z]) {
     // no observations: T = I (a = 0), the update copies the forecast anomalies
@@ -650,68 +649,6 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
   }
   // out = Sf_z T: Out^T strip = T^T Sf_strip^T, A operand X[16 J + li][k]
   const double* xr = L.t + li * LDT + 2 * kk;
-  if constexpr (F32) {
-    // mixed precision (the arithmetic of k_update_f32): b = (E - xbar)/sqrt(N-1)
-    // in FP32, xa = xbar + clamp(sum_k (E_k - xbar) a_k/sqrt(N-1)) in FP64,
-    // Ea = xa + sqrt(N-1) infl (Sf T) with the f32 MFMA D layout
-    // D[4 (lane / 16) + r][lane % 16]
-    const double sc_infl = scaling * infl;
-    const float inv = (float)(1.0 / scaling);
-    for (int64_t rb = 16 * wave; rb < sz; rb += 128) {
-      if (rb != 16 * wave) load_strip(rb);
-      const int64_t row = rb + li;
-      const bool rok = row < sz;
-      const float xb = (float)x0;
-      float(&bf)[P128 / 4] = b;  // centred in place
-      double dxp = 0.0;
-#pragma unroll
-      for (int q = 0; q < P128 / 4; ++q) {
-        const int kc = 8 * (q >> 1) + 2 * kk + (q & 1);
-        const bool ok = rok && kc < N;
-        const float ev = b[q];
-        if (ok) dxp = fma((double)ev - x0, L.a[kc], dxp);
-        bf[q] = ok ? (ev - xb) * inv : 0.0f;
-      }
-      dxp += shfl_xor_d(dxp, 16);
-      dxp += shfl_xor_d(dxp, 32);
-      double xar = x0 + dxp;
-      if (xar - maxcorr > x0) xar = x0 + maxcorr;
-      if (xar < x0 - maxcorr) xar = x0 - maxcorr;
-      // two passes over the output tiles (4 each) keep the accumulators small
-#pragma unroll 1
-      for (int h = 0; h < 2; ++h) {
-        const double* xh = xr + 64 * h * LDT;
-        f4 acc[4];
-#pragma unroll
-        for (int J = 0; J < 4; ++J) acc[J] = f4{0, 0, 0, 0};
-#pragma unroll
-        for (int p = 0; p < P128 / 8; ++p) {
-          asm volatile("" ::: "memory");
-          double2 A[4];
-#pragma unroll
-          for (int J = 0; J < 4; ++J) A[J] = *reinterpret_cast<const double2*>(xh + 16 * J * LDT + 8 * p);
-#pragma unroll
-          for (int J = 0; J < 4; ++J)
-            acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].x, bf[2 * p], acc[J], 0, 0, 0);
-#pragma unroll
-          for (int J = 0; J < 4; ++J)
-            acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)A[J].y, bf[2 * p + 1], acc[J], 0, 0, 0);
-        }
-        if (rok) {
-#pragma unroll
-          for (int J = 0; J < 4; ++J) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int col = 64 * h + 16 * J + 4 * kk + r;
-              if (col < N)
-                Out[(i0 + row) + (int64_t)col * ldO] = (float)(xar + sc_infl * (double)acc[J][r]);
-            }
-          }
-        }
-      }
-    }
-    return;
-  } else {
   for (int64_t rb = 16 * wave; rb < sz; rb += 128) {
     if (rb != 16 * wave) load_strip(rb);
     const int64_t row = rb + li;
@@ -774,7 +711,6 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
         }
       }
     }
-  }
   }
 }
 

@@ -220,6 +220,8 @@ def phase_traffic(kern, mixed, config):
         return None
     tab = json.load(open(path))
     names = PHASE_KERNELS.get(kern)
+    if kern == "update" and mixed:
+        names = ["k_xform128f"]
     if not names:
         return None
     per = tab["bytes_per_dispatch"]

@@ -177,6 +177,34 @@ def run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea, label):
     return ms, phase, st
 
 
+def run_report(args, H, Hshift, yo, rmse2, Ef, Ea):
+    """Assim's forecast and analysis reports (assimilation.F90:3389-3391) of
+    the FP64 ensembles, by oakg_ensemble_report: H·E + Hshift, observation
+    anomalies and the statistics, timed per call after one warm-up."""
+    import torch
+    from oak_amd import Context, OAKG_FP64
+    from oak_amd.assimilation import REPORT_FIELDS, ensemble_report
+    ctx = Context(0, OAKG_FP64)
+    E = Ef.double() if Ef.dtype != torch.float64 else Ef
+    ensemble_report(H, Hshift, yo, rmse2, E, ctx=ctx)
+    torch.cuda.synchronize()
+    k = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        rf = ensemble_report(H, Hshift, yo, rmse2, E, ctx=ctx)
+        ra = ensemble_report(H, Hshift, yo, rmse2, Ea, ctx=ctx)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / (2 * k)
+    ctx.close()
+    rf, ra = rf.cpu().numpy()[0], ra.cpu().numpy()[0]
+    print("# report: %.3f ms per call; forecast %s; analysis %s" % (
+        ms, {k: round(float(v), 6) for k, v in zip(REPORT_FIELDS, rf)},
+        {k: round(float(v), 6) for k, v in zip(REPORT_FIELDS, ra)}),
+        file=sys.stderr, flush=True)
+    return {"ms_per_call": ms, "forecast": dict(zip(REPORT_FIELDS, rf.tolist())),
+            "analysis": dict(zip(REPORT_FIELDS, ra.tolist()))}
+
+
 def roofline(F, phase, mixed, eig_name, config, N):
     """Dominant phase of the step and its achieved rate against the MI355X peak
     of the arithmetic it runs on (FP32 MFMA for the mixed contractions, FP64
@@ -395,6 +423,8 @@ def main():
         out["mixed_speedup_vs_fp64"] = ms64 / ms
         e = ensemble_errors(Ea32, Ea64, DP.E)
         out["mixed_err_vs_fp64"] = dict(e, bound=1e-4)
+    if Ea64 is not None:
+        out["report"] = run_report(args, H, Hshift, yo, rmse2, DP.E, Ea64)
     if args.calib:
         # a plain copy of known size: 1 GiB read + 1 GiB written
         src = torch.ones(1 << 27, dtype=torch.float64, device=device)

@@ -24,6 +24,12 @@
  *   oakg_ensemble_analysis_f32  mixed-precision variant of the above (FP32
  *                               ensembles, FP64 eigensolve); no OAK counterpart
  *                               beyond oak_assim built with PRECISION=single
+ *   oakg_report              <- report(unit, prefix, mjd, ingrid, R, HS, yo_Hx,
+ *                               exclude_obs) assimilation.F90:3584-3628 (the
+ *                               statistics; the caller writes the log lines)
+ *   oakg_ensemble_report     <- the forecast/analysis report calls of Assim's
+ *                               ensemble path, assimilation.F90:3389-3391,
+ *                               3424-3429
  *   oakg_coo_to_csr          <- SparseMatrix (matoper.F90:30-39) as used by
  *                               obsoper (assimilation.F90:2836-2898)
  *   oakg_partition           <- initPartition (assimilation.F90:578-641) after
@@ -188,8 +194,40 @@ int oakg_ensemble_analysis_f32(oakg_ctx* ctx, const oakg_zones* zones, const oak
                                const float* Ef, int64_t ldEf, float* Ea, int64_t ldEa,
                                double inflation, double max_correction, uint32_t flags);
 
-/* Host helpers ------------------------------------------------------------ */
-/* Stable COO -> CSR by row (entries of a row keep their COO order). */
+/* Innovation statistics of report (assimilation.F90:3584-3628), which Assim
+ * writes to the log for the forecast and the analysis (:3389-3391) and for
+ * each observed variable (:3410-3430).  For each observation range r,
+ * [range_ptr[r], range_ptr[r+1]) (range_ptr NULL and nranges = 1: all m),
+ * with d = yo_Hx, D the exclusion mask (0 = excluded; excluded may be NULL),
+ * R^-1 d = D (D d) / rmse2 (DCDCovar, covariance.F90:612-619) and
+ * ingrid = count(D != 0) in the range:
+ *   stats[5r+0]  ingrid
+ *   stats[5r+1]  rms_yo-Hx                  sqrt(sum(d^2, D != 0) / ingrid)
+ *   stats[5r+2]  bias_yo-Hx                 sum(d, D != 0) / ingrid
+ *   stats[5r+3]  rms_invsqrtR_yo-Hx         sqrt(sum(d R^-1 d) / ingrid)
+ *   stats[5r+4]  projection_yo-Hx_into_HSf  sqrt(|HS^T R^-1 d|^2 / ingrid)
+ * (NaN where ingrid = 0, as in OAK).  HS is m x N (ldHS), N <= 128; every
+ * rmse2 entry > 0.  The sums run in a fixed order: repeated calls give the
+ * same bits. */
+int oakg_report(oakg_ctx* ctx, int64_t m, int32_t N, const double* yo_Hx, const double* HS,
+                int64_t ldHS, const double* rmse2, const uint8_t* excluded, int32_t nranges,
+                const int64_t* range_ptr, double* stats, uint32_t flags);
+
+/* The same statistics for an ensemble E (n x N, n = H->n) as Assim's
+ * ensemble path forms them: HE = H·E + Hshift, Hx = mean(HE, 2),
+ * HS = (HE - Hx)/sqrt(N-1), d = yo - Hx (assimilation.F90:3112-3134 for the
+ * forecast Ef, :3325-3357 for the analysis Ea).  Only obs->yo, rmse2 and
+ * excluded are used.  The _f32 variant takes FP32 ensembles (the mixed
+ * path's state), with HS rounded to FP32 and the sums in FP64. */
+int oakg_ensemble_report(oakg_ctx* ctx, const oakg_obs* obs, const oakg_csr* H,
+                         const double* Hshift, int32_t N, const double* E, int64_t ldE,
+                         int32_t nranges, const int64_t* range_ptr, double* stats,
+                         uint32_t flags);
+int oakg_ensemble_report_f32(oakg_ctx* ctx, const oakg_obs* obs, const oakg_csr* H,
+                             const float* Hshift, int32_t N, const float* E, int64_t ldE,
+                             int32_t nranges, const int64_t* range_ptr, double* stats,
+                             uint32_t flags);
+
 /* Innovation negative log-likelihood of the local analysis and its gradient
    (a hand-written adjoint; no counterpart in OAK, which has no
    differentiation: it serves the tuning of the analysis hyperparameters).
@@ -210,6 +248,8 @@ int oakg_local_nll(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs, 
                    double* nll, double* grad_log_sigma, double* grad_corr_len,
                    double* grad_inflation, double* nll_zone, uint32_t flags);
 
+/* Host helpers ------------------------------------------------------------ */
+/* Stable COO -> CSR by row (entries of a row keep their COO order). */
 int oakg_coo_to_csr(int64_t m, int64_t nnz, const int32_t* row, const int32_t* col,
                     const double* val, int64_t* row_ptr, int32_t* col_out, double* val_out);
 /* Zone layout from a partition vector: returns nzones (or -1), fills

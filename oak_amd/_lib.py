@@ -23,7 +23,8 @@ EXPORTS = [
     "oakg_version", "oakg_create", "oakg_destroy", "oakg_last_error", "oakg_set_stream",
     "oakg_set_timing", "oakg_set_eigensolver", "oakg_get_stats", "oakg_get_counts", "oakg_local_analysis",
     "oakg_local_analysis_csr", "oakg_select_observations", "oakg_ensemble_analysis",
-    "oakg_ensemble_analysis_f32", "oakg_local_nll", "oakg_coo_to_csr", "oakg_partition",
+    "oakg_ensemble_analysis_f32", "oakg_local_nll", "oakg_report", "oakg_ensemble_report",
+    "oakg_ensemble_report_f32", "oakg_coo_to_csr", "oakg_partition",
 ]
 
 
@@ -116,6 +117,12 @@ def load():
     L.oakg_local_nll.argtypes = [
         _vp, C.POINTER(Zones), C.POINTER(Obs), C.c_int32, _vp, _vp, C.c_int64, C.c_double, _vp,
         _vp, _vp, _vp, _vp, C.c_uint32]
+    L.oakg_report.argtypes = [
+        _vp, C.c_int64, C.c_int32, _vp, _vp, C.c_int64, _vp, _vp, C.c_int32, _vp, _vp, C.c_uint32]
+    L.oakg_ensemble_report.argtypes = [
+        _vp, C.POINTER(Obs), C.POINTER(Csr), _vp, C.c_int32, _vp, C.c_int64, C.c_int32, _vp, _vp,
+        C.c_uint32]
+    L.oakg_ensemble_report_f32.argtypes = L.oakg_ensemble_report.argtypes
     L.oakg_coo_to_csr.argtypes = [C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]
     L.oakg_partition.restype = C.c_int64
     L.oakg_partition.argtypes = [C.c_int64, _vp, _vp, _vp, _vp]

@@ -244,3 +244,113 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
             Ef.ctypes.data, n, Ea.ctypes.data, n, float(inflation), maxc, 0)
     ctx.check(rc)
     return Ea
+
+
+REPORT_FIELDS = ("ingrid", "rms_yo-Hx", "bias_yo-Hx", "rms_invsqrtR_yo-Hx",
+                 "projection_yo-Hx_into_HSf")
+
+
+def _ranges(ranges, m):
+    rp = np.ascontiguousarray([0, m] if ranges is None else ranges, dtype=np.int64)
+    if rp.ndim != 1 or len(rp) < 2:
+        raise ValueError("ranges must hold at least two offsets")
+    return rp
+
+
+def report(yo_Hx, HS, rmse2, excluded=None, ranges=None, ctx=None):
+    """Statistics of OAK's report (assimilation.F90:3584-3628) on the GPU.
+
+    yo_Hx (m), HS (m x N), rmse2 = the diagonal of R, excluded = OAK's
+    exclude_obs (R is then wrapped as DCDCovar, assimilation.F90:3086-3092).
+    ranges: observation offsets [nranges+1] (the per-variable reports of
+    :3410-3430); default one range of all m.  Returns an array (nranges, 5)
+    with the columns of REPORT_FIELDS."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    m, N = HS.shape
+    if _is_dev(HS):
+        import torch
+        ld = _check_dev_matrix(HS, m, N, "HS") if m > 0 else 1
+        rp = torch.as_tensor(_ranges(ranges, m), device=HS.device)
+        out = torch.empty((len(rp) - 1, 5), dtype=torch.float64, device=HS.device)
+        rc = L.oakg_report(ctx.handle, m, N, _lib.ptr(yo_Hx), HS.data_ptr(), max(ld, 1),
+                           _lib.ptr(rmse2), _lib.ptr(excluded), len(rp) - 1, rp.data_ptr(),
+                           out.data_ptr(), _lib.OAKG_DEVICE_PTRS)
+        ctx.check(rc)
+        return out
+    HS = np.asfortranarray(HS, dtype=np.float64)
+    d = _host_vec(yo_Hx); r2 = _host_vec(rmse2); ex = _host_vec(excluded, np.uint8)
+    rp = _ranges(ranges, m)
+    out = np.zeros((len(rp) - 1, 5))
+    rc = L.oakg_report(ctx.handle, m, N, _lib.ptr(d), HS.ctypes.data, max(m, 1), _lib.ptr(r2),
+                       _lib.ptr(ex), len(rp) - 1, rp.ctypes.data, out.ctypes.data, 0)
+    ctx.check(rc)
+    return out
+
+
+def ensemble_report(H, Hshift, yo, rmse2, E, excluded=None, ranges=None, ctx=None):
+    """The forecast (E = Ef) or analysis (E = Ea) report of Assim's ensemble
+    path (assimilation.F90:3389-3391, 3424-3429): HE = H E + Hshift,
+    Hx = mean(HE), HS = (HE - Hx)/sqrt(N-1), d = yo - Hx.  H = (row_ptr, col,
+    val) as for ensemble_analysis; a float32 ensemble uses the FP32 entry
+    point.  Returns an array (nranges, 5) with the columns of REPORT_FIELDS."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    rp_, co, va = H
+    m = len(yo)
+    n, N = E.shape
+    if _is_dev(E):
+        import torch
+        ldE = _check_dev_matrix(E, n, N, "E")
+        o = Obs(m, None, None, None, None, _lib.ptr(yo), _lib.ptr(rmse2), _lib.ptr(excluded))
+        h = Csr(m, n, len(va), rp_.data_ptr(), co.data_ptr(), va.data_ptr())
+        if E.dtype == torch.float64:
+            fn = L.oakg_ensemble_report
+        elif E.dtype == torch.float32:
+            fn = L.oakg_ensemble_report_f32
+            Hshift = Hshift.to(torch.float32).contiguous()
+        else:
+            raise TypeError("E must be float64 or float32")
+        rr = torch.as_tensor(_ranges(ranges, m), device=E.device)
+        out = torch.empty((len(rr) - 1, 5), dtype=torch.float64, device=E.device)
+        rc = fn(ctx.handle, C.byref(o), C.byref(h), Hshift.data_ptr(), N, E.data_ptr(), ldE,
+                len(rr) - 1, rr.data_ptr(), out.data_ptr(), _lib.OAKG_DEVICE_PTRS)
+        ctx.check(rc)
+        return out
+    mixed = np.asarray(E).dtype == np.float32
+    sdt = np.float32 if mixed else np.float64
+    E = np.asfortranarray(E, dtype=sdt)
+    yo = _host_vec(yo); rmse2 = _host_vec(rmse2); Hshift = _host_vec(Hshift, sdt)
+    ex = _host_vec(excluded, np.uint8)
+    rp_ = np.ascontiguousarray(rp_, np.int64); co = np.ascontiguousarray(co, np.int32)
+    va = np.ascontiguousarray(va, np.float64)
+    o = Obs(m, None, None, None, None, yo.ctypes.data, rmse2.ctypes.data, _lib.ptr(ex))
+    h = Csr(m, n, len(va), rp_.ctypes.data, co.ctypes.data, va.ctypes.data)
+    rr = _ranges(ranges, m)
+    out = np.zeros((len(rr) - 1, 5))
+    fn = L.oakg_ensemble_report_f32 if mixed else L.oakg_ensemble_report
+    rc = fn(ctx.handle, C.byref(o), C.byref(h), Hshift.ctypes.data, N, E.ctypes.data, n,
+            len(rr) - 1, rr.ctypes.data, out.ctypes.data, 0)
+    ctx.check(rc)
+    return out
+
+
+def _fortran_e(x, w=15, d=7):
+    """Fortran's Ew.d edit descriptor (0.ddddddd E+xx), right-justified."""
+    if not np.isfinite(x):
+        return ("NaN" if np.isnan(x) else ("-Infinity" if x < 0 else "Infinity")).rjust(w)
+    if x == 0:
+        return ("0." + "0" * d + "E+00").rjust(w)
+    mant, ex = ("%.*e" % (d - 1, abs(x))).split("e")
+    body = "0." + mant.replace(".", "") + "E%+03d" % (int(ex) + 1)
+    return (("-" if x < 0 else "") + body).rjust(w)
+
+
+def write_report(unit, prefix, mjd, stats):
+    """The log lines of report (assimilation.F90:3606-3620) for one row of
+    report()/ensemble_report() output, written to the text stream ``unit``."""
+    unit.write("%smjd                      %15.4f\n" % (prefix, mjd))
+    names = ("rms_yo-Hx                ", "bias_yo-Hx               ",
+             "rms_invsqrtR_yo-Hx       ", "projection_yo-Hx_into_HSf")
+    for name, v in zip(names, list(stats)[1:5]):
+        unit.write("%s%s%s\n" % (prefix, name, _fortran_e(float(v))))

@@ -631,8 +631,9 @@ void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st) {
 
 template <int NP>
 static void prep_ens_t(const PrepEnsArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_state_mean, dim3(nblk(A.n, 256)), dim3(256), 0, st, A.n, A.N, A.E, A.ldE,
-                     A.xbar);
+  if (A.xbar)  // NULL: observation side only (the report)
+    hipLaunchKernelGGL(k_state_mean, dim3(nblk(A.n, 256)), dim3(256), 0, st, A.n, A.N, A.E,
+                       A.ldE, A.xbar);
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs, dim3(nblk(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E, A.ldE, A.Hshift, A.HEc, A.err);

@@ -110,6 +110,29 @@ struct ChunkArgs {
   double *nll_J, *nll_gL, *nll_gA, *nll_gth;
 };
 
+// Innovation statistics of report (report.hip): pieces of at most kRepRows
+// observations, each inside one observation range.
+constexpr int kRepRows = 1024;
+struct RepPiece {
+  int64_t r0, r1;
+};
+struct ReportArgs {
+  int N;
+  int layout;  // 0: HS column-major (ldHS); 1: row-major NP-padded in hs_col order
+  int64_t npieces;
+  int nranges;
+  const RepPiece* pieces;
+  const int64_t* pbeg;  // [nranges+1] first piece of each range
+  const double *d, *rmse2;
+  const uint8_t* excl;
+  const double* HS;
+  const float* HS32;  // layout 1 in FP32 (mixed path)
+  int64_t ldHS;
+  double* part;   // npieces x (4 + NP)
+  double* stats;  // nranges x 5
+  unsigned* err;
+};
+
 void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
 void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
@@ -118,6 +141,7 @@ void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_nll_adjoint(int NP, const ChunkArgs& A, hipStream_t st);
+void launch_report(int NP, const ReportArgs& A, hipStream_t st);
 void launch_nll_totals(int64_t nz, const double* Jz, const double* gA, double* out,
                        hipStream_t st);
 

@@ -798,6 +798,185 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
     });
 }
 
+// Statistics of report (assimilation.F90:3584-3628) over the observation
+// ranges [rp[r], rp[r+1]) (host array), from d, rmse2, excl and HS already on
+// the device (A's layout); writes nranges x 5 values to stats.
+void run_report(oakg_ctx* c, int NP, ReportArgs A, int32_t nranges, const int64_t* rp,
+                double* stats, bool dev) {
+  std::vector<RepPiece> pcs;
+  std::vector<int64_t> pbeg(nranges + 1);
+  for (int r = 0; r < nranges; ++r) {
+    pbeg[r] = (int64_t)pcs.size();
+    for (int64_t a = rp[r]; a < rp[r + 1]; a += kRepRows)
+      pcs.push_back(RepPiece{a, std::min<int64_t>(a + kRepRows, rp[r + 1])});
+  }
+  pbeg[nranges] = (int64_t)pcs.size();
+  const int64_t np = (int64_t)pcs.size();
+  RepPiece* dp = (RepPiece*)c->ws.get(Workspace::REP_PIECES, sizeof(RepPiece) * std::max<int64_t>(np, 1));
+  int64_t* db = (int64_t*)c->ws.get(Workspace::REP_PBEG, sizeof(int64_t) * (nranges + 1));
+  double* part = (double*)c->ws.get(Workspace::REP_PART,
+                                    sizeof(double) * (4 + NP) * std::max<int64_t>(np, 1));
+  double* dst = dev ? stats : (double*)c->ws.get(Workspace::REP_STATS, sizeof(double) * 5 * nranges);
+  if (!dp || !db || !part || !dst) check(c, hipErrorOutOfMemory, "workspace");
+  if (np > 0)
+    check(c, hipMemcpyAsync(dp, pcs.data(), sizeof(RepPiece) * np, hipMemcpyHostToDevice, c->stream),
+          "H2D pieces");
+  check(c, hipMemcpyAsync(db, pbeg.data(), sizeof(int64_t) * (nranges + 1), hipMemcpyHostToDevice,
+                          c->stream),
+        "H2D pieces");
+  A.npieces = np;
+  A.nranges = nranges;
+  A.pieces = dp;
+  A.pbeg = db;
+  A.part = part;
+  A.stats = dst;
+  A.err = c->d_err;
+  launch_report(NP, A, c->stream);
+  check(c, hipGetLastError(), "report launch");
+  finish_call(c, 0, nullptr, nullptr);  // synchronizes (the host vectors above stay alive)
+  if (!dev)
+    check(c, hipMemcpy(stats, dst, sizeof(double) * 5 * nranges, hipMemcpyDeviceToHost), "D2H stats");
+}
+
+// Observation ranges of a report call as a host array: range_ptr (nranges+1
+// offsets, a device array with dev) or one range [0, m).
+std::vector<int64_t> report_ranges(oakg_ctx* c, int64_t m, int32_t nranges,
+                                   const int64_t* range_ptr, bool dev) {
+  std::vector<int64_t> rp;
+  if (!range_ptr) {
+    if (nranges != 1) arg_fail(c, "nranges must be 1 without range_ptr");
+    rp = {0, m};
+    return rp;
+  }
+  if (nranges < 1) arg_fail(c, "nranges must be at least 1");
+  rp.resize(nranges + 1);
+  if (dev)
+    check(c, hipMemcpy(rp.data(), range_ptr, sizeof(int64_t) * (nranges + 1), hipMemcpyDeviceToHost),
+          "D2H range_ptr");
+  else
+    memcpy(rp.data(), range_ptr, sizeof(int64_t) * (nranges + 1));
+  if (rp[0] < 0 || rp[nranges] > m) arg_fail(c, "range_ptr outside [0, m]");
+  for (int r = 0; r < nranges; ++r)
+    if (rp[r + 1] < rp[r]) arg_fail(c, "range_ptr must be non-decreasing");
+  return rp;
+}
+
+int report_impl(oakg_ctx* c, int64_t m, int32_t N, const double* yo_Hx, const double* HS,
+                int64_t ldHS, const double* rmse2, const uint8_t* excluded, int32_t nranges,
+                const int64_t* range_ptr, double* stats, uint32_t flags) {
+  const bool dev = flags & OAKG_DEVICE_PTRS;
+  if (m < 0 || N < 1) arg_fail(c, "m must be >= 0 and N >= 1");
+  const int NP = padded_n(N);
+  if (!NP) {
+    c->err = "N above 128 is not supported by this build";
+    return OAKG_EUNSUP;
+  }
+  if (!stats) arg_fail(c, "stats is required");
+  if (m > 0 && (!yo_Hx || !HS || !rmse2)) arg_fail(c, "yo_Hx, HS and rmse2 are required");
+  if (m > 0 && ldHS < m) arg_fail(c, "ldHS must be at least m");
+  if (m > INT32_MAX) arg_fail(c, "m above 2^31-1");
+  begin_call(c);
+  const std::vector<int64_t> rp = report_ranges(c, m, nranges, range_ptr, dev);
+  int64_t ld = ldHS;
+  ReportArgs A{};
+  A.N = N;
+  A.layout = 0;
+  A.d = stage(c, Workspace::REP_D, yo_Hx, m, dev);
+  A.rmse2 = stage(c, Workspace::O_RMSE2, rmse2, m, dev);
+  A.excl = stage(c, Workspace::O_EXCL, excluded, m, dev);
+  A.HS = m > 0 ? stage2d(c, Workspace::REP_HS, HS, m, N, ldHS, dev, &ld) : nullptr;
+  A.ldHS = ld;
+  run_report(c, NP, A, nranges, rp.data(), stats, dev);
+  return OAKG_OK;
+}
+
+// The forecast (E = Ef) or analysis (E = Ea) report of Assim's ensemble
+// path: HE = H E + Hshift, Hx = mean(HE, 2), HS = (HE - Hx)/sqrt(N-1), d = yo - Hx
+// (assimilation.F90:3112-3134, 3325-3357), by the kernels of the analysis.
+template <typename T>
+int ensemble_report_impl(oakg_ctx* c, const oakg_obs* obs, const oakg_csr* H, const T* Hshift,
+                         int32_t N, const T* E, int64_t ldE, int32_t nranges,
+                         const int64_t* range_ptr, double* stats, uint32_t flags) {
+  constexpr bool mixed = std::is_same<T, float>::value;
+  return guarded(c, [&] {
+    const bool dev = flags & OAKG_DEVICE_PTRS;
+    if (!obs || !H || !E || !stats) arg_fail(c, "obs, H, E and stats are required");
+    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+    const int NP = padded_n(N);
+    validate_obs(c, obs, false, 1);
+    const int64_t m = obs->m, n = H->n;
+    if (H->m != m) arg_fail(c, "H->m must equal obs->m");
+    if (n < 1) arg_fail(c, "H->n must be >= 1");
+    if (m > 0 && (!H->row_ptr || (H->nnz > 0 && (!H->col || !H->val)) || !Hshift))
+      arg_fail(c, "H arrays and Hshift are required");
+    if (!dev && m > 0) {
+      if (H->row_ptr[0] != 0 || H->row_ptr[m] != H->nnz) arg_fail(c, "H->row_ptr inconsistent");
+      for (int64_t e = 0; e < H->nnz; ++e)
+        if (H->col[e] < 0 || H->col[e] >= n) arg_fail(c, "H column out of range");
+    }
+    if (ldE < std::max<int64_t>(n, 1)) arg_fail(c, "leading dimension too small");
+    begin_call(c);
+    const std::vector<int64_t> rp = report_ranges(c, m, nranges, range_ptr, dev);
+    Geo g = stage_geo(c, nullptr, obs, dev, false);
+    int64_t ld_e = 0;
+    const T* dE = stage2d(c, Workspace::IN_S, E, n, N, ldE, dev, &ld_e);
+    const int64_t* drp = stage(c, Workspace::H_RP, H->row_ptr, m + 1, dev);
+    const int32_t* dcol = stage(c, Workspace::H_COL, H->col, H->nnz, dev);
+    const double* dval = stage(c, Workspace::H_VAL, H->val, H->nnz, dev);
+    const T* dHs = stage(c, Workspace::H_SHIFT, Hshift, m, dev);
+    double* HEc = (double*)c->ws.get(Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
+    double* HS = nullptr;
+    float* HS32 = nullptr;
+    if (mixed)
+      HS32 = (float*)c->ws.get(Workspace::HS32, sizeof(float) * std::max<int64_t>(m, 1) * NP);
+    else
+      HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+    double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+    double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+    if (!HEc || !(mixed ? (void*)HS32 : (void*)HS) || !innov || !rho)
+      check(c, hipErrorOutOfMemory, "workspace");
+    PrepEnsArgs P{};
+    P.n = n;
+    P.m = m;
+    P.N = N;
+    P.ldE = ld_e;
+    P.row_ptr = drp;
+    P.col = dcol;
+    P.val = dval;
+    P.yo = g.od.yo;
+    P.rmse2 = g.od.rmse2;
+    P.excl = g.od.excluded;
+    P.scaling = sqrt((double)N - 1.0);
+    P.xbar = nullptr;  // the state mean is not needed
+    P.HEc = HEc;
+    P.HS = HS;
+    P.HS32 = HS32;
+    P.innov = innov;
+    P.rho = rho;
+    P.err = c->d_err;
+    if (mixed) {
+      P.E32 = (const float*)dE;
+      P.Hshift32 = (const float*)dHs;
+    } else {
+      P.E = (const double*)dE;
+      P.Hshift = (const double*)dHs;
+    }
+    launch_prep_ens(NP, P, c->stream);
+    check(c, hipGetLastError(), "prep");
+    ReportArgs A{};
+    A.N = N;
+    A.layout = 1;
+    A.d = innov;
+    A.rmse2 = g.od.rmse2;
+    A.excl = g.od.excluded;
+    A.HS = HS;
+    A.HS32 = HS32;
+    A.ldHS = NP;
+    run_report(c, NP, A, nranges, rp.data(), stats, dev);
+    return OAKG_OK;
+  });
+}
+
 }  // namespace
 
 extern "C" {
@@ -955,6 +1134,31 @@ int oakg_ensemble_analysis_f32(oakg_ctx* c, const oakg_zones* zones, const oakg_
                                double max_correction, uint32_t flags) {
   return ensemble_impl<float>(c, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, inflation,
                               max_correction, flags);
+}
+
+int oakg_report(oakg_ctx* c, int64_t m, int32_t N, const double* yo_Hx, const double* HS,
+                int64_t ldHS, const double* rmse2, const uint8_t* excluded, int32_t nranges,
+                const int64_t* range_ptr, double* stats, uint32_t flags) {
+  return guarded(c, [&] {
+    return report_impl(c, m, N, yo_Hx, HS, ldHS, rmse2, excluded, nranges, range_ptr, stats,
+                       flags);
+  });
+}
+
+int oakg_ensemble_report(oakg_ctx* c, const oakg_obs* obs, const oakg_csr* H,
+                         const double* Hshift, int32_t N, const double* E, int64_t ldE,
+                         int32_t nranges, const int64_t* range_ptr, double* stats,
+                         uint32_t flags) {
+  return ensemble_report_impl<double>(c, obs, H, Hshift, N, E, ldE, nranges, range_ptr, stats,
+                                      flags);
+}
+
+int oakg_ensemble_report_f32(oakg_ctx* c, const oakg_obs* obs, const oakg_csr* H,
+                             const float* Hshift, int32_t N, const float* E, int64_t ldE,
+                             int32_t nranges, const int64_t* range_ptr, double* stats,
+                             uint32_t flags) {
+  return ensemble_report_impl<float>(c, obs, H, Hshift, N, E, ldE, nranges, range_ptr, stats,
+                                     flags);
 }
 
 int oakg_local_nll(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, int32_t N,

@@ -178,8 +178,9 @@ static inline unsigned nblk32(int64_t n, int b) { return (unsigned)((n + b - 1) 
 
 template <int NP>
 static void prep_ens_f32_t(const PrepEnsArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_state_mean_f32, dim3(nblk32(A.n, 256)), dim3(256), 0, st, A.n, A.N, A.E32,
-                     A.ldE, A.xbar);
+  if (A.xbar)  // NULL: observation side only (the report)
+    hipLaunchKernelGGL(k_state_mean_f32, dim3(nblk32(A.n, 256)), dim3(256), 0, st, A.n, A.N,
+                       A.E32, A.ldE, A.xbar);
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs_f32, dim3(nblk32(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E32, A.ldE, A.Hshift32, A.HEc, A.err);

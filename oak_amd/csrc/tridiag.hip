@@ -887,7 +887,6 @@ __global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const i
   const double2* lg = rlog + rlog_off(zl, cap);
   if (tid < len) S.buf[0][tid] = lg[rlog_rec(tid)];
   __syncthreads();
-  typedef __attribute__((address_space(3))) const double lds_d;
   int64_t pos = 0;
   int b = 0;
   while (pos < len) {

@@ -100,6 +100,28 @@ module oakg
       real(c_double), value :: inflation, max_correction
       integer(c_int32_t), value :: flags
     end function
+
+    integer(c_int) function oakg_report(ctx, m, N, yo_Hx, HS, ldHS, rmse2, excluded, nranges, &
+         range_ptr, stats, flags) bind(C, name="oakg_report")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: m, ldHS
+      integer(c_int32_t), value :: N, nranges
+      type(c_ptr), value :: yo_Hx, HS, rmse2, excluded, range_ptr, stats
+      integer(c_int32_t), value :: flags
+    end function
+
+    integer(c_int) function oakg_ensemble_report(ctx, obs, H, Hshift, N, E, ldE, nranges, &
+         range_ptr, stats, flags) bind(C, name="oakg_ensemble_report")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, oakg_obs, oakg_csr
+      type(c_ptr), value :: ctx
+      type(oakg_obs), intent(in) :: obs
+      type(oakg_csr), intent(in) :: H
+      type(c_ptr), value :: Hshift, E, range_ptr, stats
+      integer(c_int32_t), value :: N, nranges
+      integer(c_int64_t), value :: ldE
+      integer(c_int32_t), value :: flags
+    end function
   end interface
 
 contains

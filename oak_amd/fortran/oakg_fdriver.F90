@@ -29,6 +29,7 @@ program oakg_fdriver
   integer :: zoneSize(nzone), i, k, l, info, u, n
   real(c_double), allocatable, target :: xf(:), Hxf(:), yo(:), Sf(:,:), HSf(:,:), Rd(:), xa(:), Sa(:,:)
   character(len=1024) :: fname
+  logical, allocatable :: excl(:)
 
   call get_command_argument(1, fname)
   n = nzone * zsize
@@ -52,6 +53,13 @@ program oakg_fdriver
   Sa = 0
   call locAnalysis_gpu(zoneSize, select_gauss, xf, Hxf, yo, Sf, HSf, Rd, xa, Sa, info=info)
   print '(a,i0)', 'locAnalysis_gpu info = ', info
+  ! the forecast report of Assim (assimilation.F90:3389), the last
+  ! observation excluded
+  allocate(excl(nobs))
+  excl = .false.
+  excl(nobs) = .true.
+  if (info == 0) &
+    call report_gpu(6, 'forecast.', 55000.5d0, count(.not. excl), Rd, HSf, yo - Hxf, excl)
   if (len_trim(fname) > 0) then
     open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
     write(u) int(info, 8), int(nzone, 8), int(zsize, 8), int(nmem, 8), int(nobs, 8)

@@ -13,7 +13,7 @@ module oakg_rrsqrt
   use oakg
   implicit none
   private
-  public :: locAnalysis_gpu, oakg_rrsqrt_context
+  public :: locAnalysis_gpu, report_gpu, oakg_rrsqrt_context
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -114,4 +114,51 @@ contains
 !$omp end master
 !$omp barrier
   end subroutine locAnalysis_gpu
+
+  ! report (assimilation.F90:3584-3628) with the statistics computed on the
+  ! GPU: the same arguments, with R given by its diagonal Rdiag (R%D), and the
+  ! same log lines.  ingrid is the divisor, as in OAK (the caller passes
+  ! count(.not.exclude_obs)); the excluded observations drop out of the sums
+  ! as with OAK's DCDCovar-wrapped R (assimilation.F90:3086-3092).
+  subroutine report_gpu(unit, prefix, mjd, ingrid, Rdiag, HS, yo_Hx, exclude_obs)
+    integer, intent(in) :: unit
+    character(len=*), intent(in) :: prefix
+    real(8), intent(in) :: mjd
+    integer, intent(in) :: ingrid
+    real(c_double), intent(in), target, contiguous :: Rdiag(:), HS(:,:), yo_Hx(:)
+    logical, intent(in) :: exclude_obs(:)
+    character(len=*), parameter :: form = '(A,A,E15.7)'
+    integer(c_int8_t), allocatable, target :: ex(:)
+    real(c_double), target :: st(5)
+    real(c_double) :: g
+    type(c_ptr) :: c
+    integer(c_int) :: rc
+    allocate(ex(size(yo_Hx)))
+    ex = 0
+    where (exclude_obs) ex = 1
+    c = oakg_rrsqrt_context()
+    rc = OAKG_EHIP
+    if (c_associated(c)) &
+      rc = oakg_report(c, int(size(yo_Hx), c_int64_t), int(size(HS, 2), c_int32_t), &
+           c_loc(yo_Hx), c_loc(HS), int(max(size(HS, 1), 1), c_int64_t), c_loc(Rdiag), c_loc(ex), &
+           1_c_int32_t, c_null_ptr, c_loc(st), 0_c_int32_t)
+    if (rc /= OAKG_OK) then
+      if (c_associated(c)) write(0, *) 'report_gpu: ', oakg_error_string(c)
+      error stop 1
+    end if
+    ! the library divides by count(.not.exclude_obs); another ingrid of the
+    ! caller rescales the sums
+    g = real(ingrid, c_double)
+    if (g /= st(1)) then
+      st(2) = sqrt(st(2)**2 * st(1) / g)
+      st(3) = st(3) * st(1) / g
+      st(4) = sqrt(st(4)**2 * st(1) / g)
+      st(5) = sqrt(st(5)**2 * st(1) / g)
+    end if
+    write(unit,'(A,A,F15.4)') prefix,'mjd                      ',mjd
+    write(unit,form) prefix,'rms_yo-Hx                ',st(2)
+    write(unit,form) prefix,'bias_yo-Hx               ',st(3)
+    write(unit,form) prefix,'rms_invsqrtR_yo-Hx       ',st(4)
+    write(unit,form) prefix,'projection_yo-Hx_into_HSf',st(5)
+  end subroutine report_gpu
 end module oakg_rrsqrt

@@ -23,6 +23,11 @@
  *                               R_loc⁻¹ = R⁻¹∘wwᵀ (test/test_rrsqrt.F90:142-236)
  *   oko_obsoper                 dense H·x brute force
  *   oko_ensemble_analysis       mean(Ea)=xa, cov(Ea)=Sa·Saᵀ, empty zones = identity
+ *   oko_report                  hand-computed known answer; scaling of d; an
+ *                               excluded observation = a removed one; sums over
+ *                               observation ranges add up to the whole
+ *   oko_ensemble_report         projection^2 ingrid = r^T cov(HE) r with numpy's
+ *                               sample covariance, d = yo - mean(HE)
  */
 #define _GNU_SOURCE
 #include <dlfcn.h>
@@ -634,6 +639,85 @@ int oko_ensemble_analysis(int nz, const int64_t* zone_ptr, int z0, int z1, int N
 done:
   free(HE); free(xf); free(Hxf); free(Sf); free(xa);
   return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* report (assimilation.F90:3584-3628): innovation statistics written to the  */
+/* log for the forecast and the analysis (:3389-3391) and per observed        */
+/* variable (:3410-3430).  d = yo - Hx (m), HS m x N (ldHS), sig2 the diagonal */
+/* of R; dex (may be NULL) is the exclusion mask D (0 = excluded), which wraps */
+/* R as DCDCovar (:3086-3092) so that R^-1 x = D (D x) / sig2                 */
+/* (covariance.F90:612-619, 425-432).  With ingrid = count(D != 0):           */
+/*   out[0] ingrid                                                            */
+/*   out[1] rms_yo-Hx                 sqrt(sum(d^2, D != 0) / ingrid)  (:3607) */
+/*   out[2] bias_yo-Hx                sum(d, D != 0) / ingrid          (:3608) */
+/*   out[3] rms_invsqrtR_yo-Hx        sqrt(sum(d * R^-1 d) / ingrid)   (:3617) */
+/*   out[4] projection_yo-Hx_into_HSf sqrt(sum((HS^T R^-1 d)^2)/ingrid) (:3620) */
+/* ------------------------------------------------------------------------- */
+void oko_report(int64_t m, int N, const double* d, const double* HS, int64_t ldHS,
+                const double* sig2, const double* dex, double* out) {
+  int64_t ingrid = 0;
+  double s2 = 0, s1 = 0, sr = 0, pp = 0;
+  for (int64_t l = 0; l < m; ++l) {
+    const int valid = !dex || dex[l] != 0;
+    if (valid) {
+      ingrid++;
+      s2 = s2 + d[l] * d[l];
+      s1 = s1 + d[l];
+    }
+  }
+  double* r = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+  if (!r) { for (int i = 0; i < 5; ++i) out[i] = NAN; return; }
+  for (int64_t l = 0; l < m; ++l) {
+    r[l] = dex ? dex[l] * ((dex[l] * d[l]) / sig2[l]) : d[l] / sig2[l];
+    sr = sr + d[l] * r[l];
+  }
+  for (int k = 0; k < N; ++k) {
+    double t = 0;
+    for (int64_t l = 0; l < m; ++l) t = t + HS[l + (size_t)k * ldHS] * r[l];
+    pp = pp + t * t;
+  }
+  free(r);
+  const double g = (double)ingrid;
+  out[0] = g;
+  out[1] = sqrt(s2 / g);
+  out[2] = s1 / g;
+  out[3] = sqrt(sr / g);
+  out[4] = sqrt(pp / g);
+}
+
+/* The forecast or analysis report of Assim's ensemble path for an ensemble E  */
+/* (n x N, ld n): HE = H E + Hshift, Hx = mean(HE, 2), HS = (HE - Hx)/sqrt(N-1) */
+/* (assimilation.F90:3112-3134 for Ef, :3325-3357 for Ea), d = yo - Hx; then   */
+/* oko_report over each observation range [range_ptr[r], range_ptr[r+1])       */
+/* (R%sub and the row slices of :3424-3429); out is 5 x nranges.              */
+int oko_ensemble_report(int64_t n, int N, int64_t m, int64_t nnz, const int32_t* Hi,
+                        const int32_t* Hj, const double* Hs, const double* Hshift,
+                        const double* yo, const double* sig2, const double* dex, const double* E,
+                        int nranges, const int64_t* range_ptr, double* out) {
+  const double scaling = sqrt((double)N - 1.);
+  double* HE = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1) * N);
+  double* Hx = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+  double* d = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+  if (!HE || !Hx || !d) { free(HE); free(Hx); free(d); return OKO_ENOMEM; }
+  for (int k = 0; k < N; ++k) {
+    oko_obsoper(m, nnz, Hi, Hj, Hs, E + (size_t)k * n, HE + (size_t)k * m);
+    for (int64_t l = 0; l < m; ++l) HE[l + (size_t)k * m] = HE[l + (size_t)k * m] + Hshift[l];
+  }
+  for (int64_t l = 0; l < m; ++l) {
+    double t = 0;
+    for (int k = 0; k < N; ++k) t += HE[l + (size_t)k * m];
+    Hx[l] = t / N;
+    d[l] = yo[l] - Hx[l];
+  }
+  for (int k = 0; k < N; ++k)
+    for (int64_t l = 0; l < m; ++l) HE[l + (size_t)k * m] = (HE[l + (size_t)k * m] - Hx[l]) / scaling;
+  for (int r = 0; r < nranges; ++r) {
+    const int64_t a = range_ptr[r], b = range_ptr[r + 1];
+    oko_report(b - a, N, d + a, HE + a, m, sig2 + a, dex ? dex + a : 0, out + 5 * r);
+  }
+  free(HE); free(Hx); free(d);
+  return OKO_OK;
 }
 
 static int cmp_i64(const void* a, const void* b) {

@@ -273,6 +273,42 @@ def ensemble_analysis(geo, H, Hshift, yo, sig2, E, dex=None, inflation=1.0,
     return Ea, xa, counts
 
 
+def report(d, HS, sig2, dex=None):
+    """report (assimilation.F90:3584-3628) for innovations d (m), HS (m x N)
+    and R = diag(sig2), optionally wrapped by the exclusion mask dex
+    (0 = excluded).  Returns [ingrid, rms, bias, rms_invsqrtR, projection]."""
+    d = _f(d)
+    HS = np.asfortranarray(HS, dtype=np.float64)
+    m, N = HS.shape
+    out = np.zeros(5)
+    lib().oko_report(C.c_int64(m), C.c_int(N), _ptr(d), _ptr(HS), C.c_int64(max(m, 1)),
+                     _ptr(_f(sig2)), _ptr(None if dex is None else _f(dex)), _ptr(out))
+    return out
+
+
+def ensemble_report(H, Hshift, yo, sig2, E, dex=None, ranges=None):
+    """The forecast (E = Ef) or analysis (E = Ea) report of Assim's ensemble
+    path (assimilation.F90:3112-3134, 3325-3357, 3389-3391, 3424-3429).
+    H = (Hi, Hj, Hs) COO, 0-based.  ranges: observation offsets [r+1]
+    (default: one range of all m).  Returns an array (nranges, 5)."""
+    Hi = np.ascontiguousarray(H[0], dtype=np.int32)
+    Hj = np.ascontiguousarray(H[1], dtype=np.int32)
+    Hs = _f(H[2])
+    E = _f(E)
+    n, N = E.shape
+    m = len(yo)
+    rp = np.ascontiguousarray([0, m] if ranges is None else ranges, dtype=np.int64)
+    nr = len(rp) - 1
+    out = np.zeros((nr, 5))
+    rc = lib().oko_ensemble_report(
+        C.c_int64(n), C.c_int(N), C.c_int64(m), C.c_int64(len(Hs)), _ptr(Hi), _ptr(Hj),
+        _ptr(Hs), _ptr(_f(Hshift)), _ptr(_f(yo)), _ptr(_f(sig2)),
+        _ptr(None if dex is None else _f(dex)), _ptr(E), C.c_int(nr), _ptr(rp), _ptr(out))
+    if rc:
+        raise OracleError("ensemble_report rc=%d" % rc)
+    return out
+
+
 def partition(part):
     """initPartition (assimilation.F90:578-641) after the gap relabelling of
     :408-412.  Returns (zone_size, zone_index, inv_zone_index), 0-based."""

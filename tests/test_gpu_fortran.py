@@ -1,6 +1,6 @@
 """Fortran host code calling the GPU path through the iso_c_binding layer
-(oak_amd/fortran): locAnalysis_gpu, with a selectObservations callback, against
-the oracle.  Needs an MI355X."""
+(oak_amd/fortran): locAnalysis_gpu, with a selectObservations callback, and
+report_gpu, against the oracle.  Needs an MI355X."""
 import os
 import subprocess
 
@@ -23,3 +23,26 @@ def test_fortran_locanalysis_vs_oracle(tmp_path):
     dx = np.abs(d["xa"] - xa).max() / np.abs(xa - d["xf"]).max()
     ds = np.abs(d["Sa"] - Sa).max() / np.abs(Sa).max()
     assert dx < 1e-10 and ds < 1e-10, (dx, ds)
+
+
+def test_fortran_report_vs_oracle(tmp_path):
+    """report_gpu's log lines (assimilation.F90:3606-3620 formats) for the
+    driver's forecast, last observation excluded."""
+    from oracle import oracle as O
+    out = tmp_path / "fd.bin"
+    r = subprocess.run([DRIVER, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = read(out)
+    vals = {}
+    for line in r.stdout.splitlines():
+        if line.startswith("forecast."):
+            key, v = line[len("forecast."):].split()
+            vals[key] = float(v)
+    assert vals["mjd"] == 55000.5
+    dex = np.ones(d["m"])
+    dex[-1] = 0
+    exp = O.report(d["yo"] - d["Hxf"], d["HSf"], d["Rd"], dex)
+    got = [vals["rms_yo-Hx"], vals["bias_yo-Hx"], vals["rms_invsqrtR_yo-Hx"],
+           vals["projection_yo-Hx_into_HSf"]]
+    # E15.7: seven significant digits
+    np.testing.assert_allclose(got, exp[1:], rtol=1e-6)

@@ -298,3 +298,95 @@ def test_ensemble_analysis_geometry_against_dense_kf(backend):
         xa_z, Pa_z = local_kf_rows(xf, Sf, Hd, yo, sig2, [z], idx, w)
         assert abs(xa[z] - xa_z[0]) < 1e-10 * max(1, abs(xa_z[0]))
         assert abs(Sa[z] @ Sa[z] - Pa_z[0, 0]) < 1e-10 * max(1, Pa_z[0, 0])
+
+
+# --- report, assimilation.F90:3584-3628 ---
+def test_report_known_answer():
+    # d = [1, 2, -2], sig2 = [1, 4, 2], third observation excluded:
+    # ingrid 2, rms sqrt(5/2), bias 3/2, R^-1 d = [1, 1/2, 0],
+    # d.R^-1 d = 2, HS^T R^-1 d = [1, 1/2]
+    d = np.array([1.0, 2.0, -2.0])
+    sig2 = np.array([1.0, 4.0, 2.0])
+    dex = np.array([1.0, 1.0, 0.0])
+    HS = np.array([[1.0, 0.0], [0.0, 1.0], [5.0, 5.0]])
+    out = O.report(d, HS, sig2, dex)
+    np.testing.assert_allclose(out, [2.0, math.sqrt(2.5), 1.5, 1.0, math.sqrt(0.625)],
+                               rtol=1e-15)
+    # nothing excluded: plain DiagCovar
+    out = O.report(d, HS, sig2)
+    r = d / sig2
+    np.testing.assert_allclose(
+        out, [3.0, math.sqrt(9 / 3), 1.0 / 3, math.sqrt((1 + 1 + 2) / 3),
+              math.sqrt(((r[0] + 5 * r[2]) ** 2 + (r[1] + 5 * r[2]) ** 2) / 3)], rtol=1e-15)
+
+
+def _report_case(seed, m=300, N=12):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=m)
+    HS = rng.normal(size=(m, N))
+    sig2 = rng.uniform(0.2, 3.0, size=m)
+    dex = (rng.random(m) > 0.2).astype(float)
+    return d, HS, sig2, dex
+
+
+def test_report_scaling_of_the_innovation():
+    d, HS, sig2, dex = _report_case(1)
+    a = O.report(d, HS, sig2, dex)
+    b = O.report(-3.0 * d, HS, sig2, dex)
+    np.testing.assert_allclose(b, a * [1, 3, -3, 3, 3], rtol=1e-13)
+
+
+def test_report_excluded_observation_equals_removed():
+    d, HS, sig2, dex = _report_case(2)
+    keep = dex != 0
+    a = O.report(d, HS, sig2, dex)
+    b = O.report(d[keep], HS[keep], sig2[keep])
+    np.testing.assert_allclose(a, b, rtol=1e-13)
+
+
+def test_report_ranges_add_up():
+    # checksum of checksums: ingrid * stat^2 is additive over observation
+    # ranges for rms, rms_invsqrtR; ingrid * bias as well
+    d, HS, sig2, dex = _report_case(3, m=500)
+    cut = [0, 123, 124, 380, 500]
+    parts = np.array([O.report(d[a:b], HS[a:b], sig2[a:b], dex[a:b])
+                      for a, b in zip(cut[:-1], cut[1:])])
+    tot = O.report(d, HS, sig2, dex)
+    g = parts[:, 0]
+    assert g.sum() == tot[0]
+    np.testing.assert_allclose((g * parts[:, 1] ** 2).sum(), tot[0] * tot[1] ** 2, rtol=1e-13)
+    np.testing.assert_allclose((g * parts[:, 2]).sum(), tot[0] * tot[2], rtol=1e-12)
+    np.testing.assert_allclose((g * parts[:, 3] ** 2).sum(), tot[0] * tot[3] ** 2, rtol=1e-13)
+
+
+def test_report_all_excluded_is_nan():
+    d, HS, sig2, _ = _report_case(4, m=10)
+    out = O.report(d, HS, sig2, np.zeros(10))
+    assert out[0] == 0 and np.all(np.isnan(out[1:]))
+
+
+def test_ensemble_report_against_sample_covariance():
+    rng = np.random.default_rng(5)
+    n, N, m = 60, 9, 40
+    E = rng.normal(size=(n, N)) + np.linspace(0, 3, n)[:, None]
+    Hi = np.repeat(np.arange(m), 2)
+    Hj = rng.integers(0, n, size=2 * m)
+    Hs = rng.uniform(0.2, 1.0, size=2 * m)
+    Hshift = rng.normal(size=m) * 0.1
+    yo = rng.normal(size=m)
+    sig2 = rng.uniform(0.5, 2.0, size=m)
+    dex = (rng.random(m) > 0.25).astype(float)
+    Hd = np.zeros((m, n))
+    np.add.at(Hd, (Hi, Hj), Hs)
+    HE = Hd @ E + Hshift[:, None]
+    dd = yo - HE.mean(axis=1)
+    cut = [0, 17, 40]
+    out = O.ensemble_report((Hi, Hj, Hs), Hshift, yo, sig2, E, dex, ranges=cut)
+    for k, (a, b) in enumerate(zip(cut[:-1], cut[1:])):
+        v = dex[a:b] != 0
+        g = v.sum()
+        r = np.where(v, dd[a:b] / sig2[a:b], 0.0)
+        P = np.cov(HE[a:b])  # sample covariance (ddof 1) = HS HS^T
+        exp = [g, math.sqrt((dd[a:b][v] ** 2).sum() / g), dd[a:b][v].sum() / g,
+               math.sqrt((dd[a:b] * r).sum() / g), math.sqrt(r @ P @ r / g)]
+        np.testing.assert_allclose(out[k], exp, rtol=1e-11)

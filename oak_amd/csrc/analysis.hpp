@@ -112,7 +112,7 @@ struct ChunkArgs {
 
 // Innovation statistics of report (report.hip): pieces of at most kRepRows
 // observations, each inside one observation range.
-constexpr int kRepRows = 1024;
+constexpr int kRepRows = 512;
 struct RepPiece {
   int64_t r0, r1;
 };

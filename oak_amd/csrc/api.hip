@@ -168,13 +168,34 @@ const T* stage2d(oakg_ctx* c, Workspace::Slot s, const T* p, int64_t rows, int c
   }
   T* d = (T*)c->ws.get(s, sizeof(T) * std::max<int64_t>(rows, 1) * cols);
   if (!d) check(c, hipErrorOutOfMemory, "staging");
-  if (rows > 0)
+  if (rows > 0 && ld == rows)
+    check(c, hipMemcpyAsync(d, p, sizeof(T) * rows * cols, hipMemcpyHostToDevice, c->stream),
+          "H2D 2D");
+  else if (rows > 0)
     check(c,
           hipMemcpy2DAsync(d, sizeof(T) * rows, p, sizeof(T) * ld, sizeof(T) * rows, cols,
                            hipMemcpyHostToDevice, c->stream),
           "H2D 2D");
   *ld_out = rows;
   return d;
+}
+
+// Copies a dense rows x cols device block into a host block with leading
+// dimension ld: one 1-D copy when the host block is dense too, else one copy
+// per column.  (hipMemcpy2D from device to host has returned success without
+// copying anything under rocprofv3's kernel tracing.)
+template <typename T>
+void copy_out_2d(oakg_ctx* c, T* dst, int64_t ld, const T* src, int64_t rows, int cols,
+                 const char* what) {
+  if (rows <= 0 || cols <= 0) return;
+  if (ld == rows) {
+    check(c, hipMemcpy(dst, src, sizeof(T) * rows * cols, hipMemcpyDeviceToHost), what);
+    return;
+  }
+  for (int k = 0; k < cols; ++k)
+    check(c, hipMemcpy(dst + (size_t)k * ld, src + (size_t)k * rows, sizeof(T) * rows,
+                       hipMemcpyDeviceToHost),
+          what);
 }
 
 struct Geo {
@@ -543,11 +564,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   (void)total;
   if (!dev) {
     check(c, hipMemcpy(xa, dxa, sizeof(double) * n, hipMemcpyDeviceToHost), "D2H xa");
-    if (n > 0)
-      check(c,
-            hipMemcpy2D(Sa, sizeof(double) * ldSa, dSa, sizeof(double) * n, sizeof(double) * n, N,
-                        hipMemcpyDeviceToHost),
-            "D2H Sa");
+    copy_out_2d(c, Sa, ldSa, dSa, n, N, "D2H Sa");
     if (ampl)
       check(c, hipMemcpy(ampl, d_ampl, sizeof(double) * (size_t)N * nz, hipMemcpyDeviceToHost),
             "D2H ampl");
@@ -789,11 +806,7 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     collect_stats(c, nz, sel.ptr, ms);
-    if (!dev && n > 0)
-      check(c,
-            hipMemcpy2D(Ea, sizeof(T) * ldEa, dEa, sizeof(T) * n, sizeof(T) * n, N,
-                        hipMemcpyDeviceToHost),
-            "D2H Ea");
+    if (!dev) copy_out_2d(c, Ea, ldEa, (const T*)dEa, n, N, "D2H Ea");
     return OAKG_OK;
     });
 }

@@ -10,14 +10,15 @@
 // One pass over the observations: each workgroup takes a piece of at most
 // kRepRows observations of one range, keeps R^-1 d of the piece in LDS and
 // writes 4 + NP partial sums; a second kernel adds the pieces of each range
-// in order (the result does not depend on the launch) and forms the
-// statistics.  The pass reads HS once (m N values): it is bound by HBM.
+// and forms the statistics.  The pass reads HS once (m N values): it is bound by HBM.
 //   layout 0: HS column-major with leading dimension ldHS (caller's HS);
 //             lanes run over observations (coalesced), wave w takes the
 //             columns k = w + 4 c
 //   layout 1: HS row-major, NP values per observation in hs_col<NP> order
 //             (the ensemble path's observation anomalies, FP64 or FP32);
 //             lanes run over the columns of a row, wave w over rows w + 4 i
+// The partial sums of the pieces are added per range in a fixed order, so
+// the bits of the result do not depend on the launch.
 #include "../../include/oakg.h"
 #include "analysis.hpp"
 #include "common.hpp"
@@ -71,39 +72,63 @@ __global__ __launch_bounds__(256) void k_report_part(const RepPiece* __restrict_
     sc[w][3] = sr;
   }
   __syncthreads();
+  // The loads of an iteration do not depend on a branch (out-of-range rows
+  // and columns read a valid element and add nothing), so each wave keeps
+  // all of them in flight at once.
   if (LAYOUT == 0) {
+    // the columns of a wave in passes of CC (CC accumulators and CC loads in
+    // flight per lane: few enough registers for six waves per SIMD)
     constexpr int NC = NP / 4;
-    double acc[NC];
+    constexpr int CC = NC < 16 ? NC : 16;
+#pragma unroll 1
+    for (int c0 = 0; c0 < NC; c0 += CC) {
+      double acc[CC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-    for (int i = lane; i < nr; i += 64) {
-      const double ri = rr[i];
-      const int64_t l = pc.r0 + i;
+      for (int c = 0; c < CC; ++c) acc[c] = 0.0;
+      for (int i0 = 0; i0 < nr; i0 += 64) {
+        const int i = i0 + lane;
+        const bool ok = i < nr;
+        const double ri = ok ? rr[i] : 0.0;
+        const int64_t l = pc.r0 + (ok ? i : 0);
+        double v[CC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int k = w + 4 * c;
-        if (k < N) acc[c] = acc[c] + (double)HS[l + (int64_t)k * ldHS] * ri;
+        for (int c = 0; c < CC; ++c) {
+          const int k = min(w + 4 * (c0 + c), N - 1);
+          v[c] = (double)HS[l + (int64_t)k * ldHS];
+        }
+#pragma unroll
+        for (int c = 0; c < CC; ++c) acc[c] = acc[c] + (ok ? v[c] * ri : 0.0);
       }
-    }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      double v = acc[c];
-      for (int o = 32; o > 0; o >>= 1) v = v + shfl_xor_d(v, o);
-      if (lane == 0) red[0][w + 4 * c] = v;
+      for (int c = 0; c < CC; ++c) {
+        double v = acc[c];
+        for (int o = 32; o > 0; o >>= 1) v = v + shfl_xor_d(v, o);
+        if (lane == 0) red[0][w + 4 * (c0 + c)] = v;
+      }
     }
   } else {
     constexpr int NH = (NP + 63) / 64;
+    constexpr int U = NP >= 64 ? 8 : 16;  // rows per iteration of a wave
     double acc[NH];
 #pragma unroll
     for (int h = 0; h < NH; ++h) acc[h] = 0.0;
-    for (int i = w; i < nr; i += 4) {
-      const double ri = rr[i];
-      const T* row = HS + (pc.r0 + i) * NP;
+    const int kl = min(lane, NP - 1);
+    for (int i0 = w; i0 < nr; i0 += 4 * U) {
+      double v[U][NH], ri[U];
+      bool ok[U];
 #pragma unroll
-      for (int h = 0; h < NH; ++h) {
-        const int k = lane + 64 * h;
-        if (k < NP) acc[h] = acc[h] + (double)row[k] * ri;
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + 4 * u;
+        ok[u] = i < nr;
+        ri[u] = ok[u] ? rr[i] : 0.0;
+        const T* row = HS + (pc.r0 + (ok[u] ? i : 0)) * NP;
+#pragma unroll
+        for (int h = 0; h < NH; ++h) v[u][h] = (double)row[kl + 64 * h];
       }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int h = 0; h < NH; ++h) acc[h] = acc[h] + (ok[u] ? v[u][h] * ri[u] : 0.0);
     }
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
@@ -126,18 +151,45 @@ __global__ __launch_bounds__(256) void k_report_part(const RepPiece* __restrict_
   if (t < 4) out[t] = ((sc[0][t] + sc[1][t]) + sc[2][t]) + sc[3][t];
 }
 
+// One workgroup per range: 8 segments of its pieces x 128 columns, each
+// thread with eight interleaved partial sums (independent loads); the
+// segments are then combined in a fixed order.
 template <int NP>
-__global__ __launch_bounds__(256) void k_report_finish(const int64_t* __restrict__ pbeg, int N,
-                                                       const double* __restrict__ part,
-                                                       double* __restrict__ stats) {
+__global__ __launch_bounds__(1024) void k_report_finish(const int64_t* __restrict__ pbeg, int N,
+                                                        const double* __restrict__ part,
+                                                        double* __restrict__ stats) {
 #pragma clang fp contract(off)
-  __shared__ double v[4 + NP];
+  constexpr int W = 4 + NP;  // partials per piece
+  __shared__ double seg[8][W];
+  __shared__ double v[W];
   const int r = blockIdx.x, t = threadIdx.x;
+  const int cl = t & 127, sg = t >> 7;
   const int64_t p0 = pbeg[r], p1 = pbeg[r + 1];
-  if (t < 4 + N) {
-    double s = 0;
-    for (int64_t p = p0; p < p1; ++p) s = s + part[p * (4 + NP) + t];
-    v[t] = s;
+  const int64_t len = (p1 - p0 + 7) / 8;
+  const int64_t a = p0 + sg * len, b = min(p1, a + len);
+  for (int col = cl; col < W; col += 128) {
+    double s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = 0.0;
+    int64_t p = a;
+    for (; p + 8 <= b; p += 8) {
+      double x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = part[(p + j) * W + col];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[j] = s[j] + x[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+      if (p + j < b) s[j] = s[j] + part[(p + j) * W + col];
+    seg[sg][col] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+  }
+  __syncthreads();
+  if (t < W) {
+    double x = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x = x + seg[j][t];
+    v[t] = x;
   }
   __syncthreads();
   if (t == 0) {
@@ -168,7 +220,7 @@ static void report_t(const ReportArgs& A, hipStream_t st) {
                          st, A.pieces, A.N, A.d, A.rmse2, A.excl, A.HS, (int64_t)NP, A.part,
                          A.err);
   }
-  hipLaunchKernelGGL(k_report_finish<NP>, dim3((unsigned)A.nranges), dim3(256), 0, st, A.pbeg,
+  hipLaunchKernelGGL(k_report_finish<NP>, dim3((unsigned)A.nranges), dim3(1024), 0, st, A.pbeg,
                      A.N, A.part, A.stats);
 }
 

@@ -340,6 +340,40 @@ def test_device_pointers_match_host(oak, cfg0):
     np.testing.assert_array_equal(Ea_d.cpu().numpy(), Ea_h)
 
 
+def test_host_leading_dimensions(oak, cfg0):
+    """Host ensembles with leading dimension n + 3 through the C ABI (staged
+    by a strided copy, written back column by column) give the bits of the
+    dense call."""
+    import ctypes as C
+    from oak_amd import _lib
+    from oak_amd.rrsqrt import default_context
+    _, A = oak
+    P = cfg0
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    g = _geometry(P)
+    Ea = A.ensemble_analysis(g, H, P.Hshift, P.yo, P.rmse2, P.E)
+    n, N = P.E.shape
+    ld = n + 3
+    Ef = np.full((ld, N), np.nan, order="F")
+    Ef[:n] = P.E
+    Eo = np.full((ld, N), -7.0, order="F")
+    gh = g.host_arrays()
+    yo = np.ascontiguousarray(P.yo, np.float64)
+    r2 = np.ascontiguousarray(P.rmse2, np.float64)
+    z, o = gh.structs(yo, r2)
+    rp, co, va = (np.ascontiguousarray(H[0], np.int64), np.ascontiguousarray(H[1], np.int32),
+                  np.ascontiguousarray(H[2], np.float64))
+    h = _lib.Csr(P.m, n, len(va), rp.ctypes.data, co.ctypes.data, va.ctypes.data)
+    hs = np.ascontiguousarray(P.Hshift, np.float64)
+    ctx = default_context()
+    rc = _lib.load().oakg_ensemble_analysis(ctx.handle, C.byref(z), C.byref(o), C.byref(h),
+                                            hs.ctypes.data, N, Ef.ctypes.data, ld,
+                                            Eo.ctypes.data, ld, 1.0, float("inf"), 0)
+    ctx.check(rc)
+    np.testing.assert_array_equal(Eo[:n], Ea)
+    assert np.all(Eo[n:] == -7.0)  # rows beyond n untouched
+
+
 def test_deterministic(oak, cfg0):
     _, A = oak
     P = cfg0

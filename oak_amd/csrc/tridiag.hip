@@ -20,8 +20,9 @@
 //                  Y = U diag(mu^-1/4), with mu = max(lambda, 1)
 //                  (rrsqrt.F90:137-142, 176-177); mu itself is kept beside a
 //                  and v for the likelihood.
-//   k_tql_apply128 (N > 64) the rotations on rows held in registers, then
-//                  the epilogue of k_eigvec_epi.
+//   k_tql_apply128 (N > 64) the rotations on rows held in registers (each
+//                  row split over two threads), then the epilogue of
+//                  k_eigvec_epi.
 #include <type_traits>
 
 #include "analysis.hpp"
@@ -826,293 +827,339 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
 }
 
 // ---------------------------------------------------------------------------
-// U = Q Z for NP = 128 with the rows in registers.  Thread r holds columns
-// 64..127 of row r in zr and 0..63 in zw (VGPRs, with part of zw in AGPRs:
-// one wave per SIMD, two zones per CU).  A chain (one QL sweep, from its
-// first position m-1 down to l) runs as unrolled, branch-free segments of 16
-// positions, each segment it meets with the identity at the positions it
-// does not cover (a jump into the middle of an unrolled sequence is not
-// structured control flow, and the compiler's rewriting of it copies the
-// registers around); the (c, s) of a segment come from a per-wave,
-// position-indexed table in LDS.  Position 63 couples the two halves.  The
-// arithmetic of every rotation is that of k_tql_apply.  The LDS (66 KB) is
-// used by the epilogue, to stage a half of U for its column sums.
+// U = Q Z for NP = 128 with the rows in registers.  A chain (one QL sweep,
+// from its first position m-1 down to l) runs as unrolled, branch-free
+// segments of 16 positions, each segment it meets with the identity at the
+// positions it does not cover (a jump into the middle of an unrolled sequence
+// is not structured control flow, and the compiler's rewriting of it copies
+// the registers around); the (c, s) of a segment come from a per-wave,
+// position-indexed table in LDS.  The arithmetic of every rotation is that of
+// k_tql_apply.  The LDS of the rotation phase is reused by the load of Q and
+// by the epilogue, to stage a half of U for its column sums.
 // ---------------------------------------------------------------------------
+// k_tql_apply128 (N > 64): the logged rotations applied to the rows of Q,
+// then the epilogue of k_eigvec_epi.  Each row of Q is split over two
+// threads, row r's columns 0..63 in thread r (waves 0-1, the lower group) and
+// columns 64..127 in thread 128 + r (waves 2-3, the upper group): 64 values a
+// thread, all in VGPRs.  A chain of rotations (positions m-1 down to l) acts
+// on the upper columns (positions >= 64), couples columns 63 and 64 (position
+// 63), then acts on the lower columns (positions <= 62).  The groups run one
+// chain apart: in step t the upper group applies chain t while the lower
+// group applies chain t-1, both after the coupling rotation of chain t-1,
+// which each group computes from its own column and the other group's column
+// published at the end of step t-1 (same inputs and operations in both
+// groups, so the two copies agree bit for bit).  The records of chains t+1,
+// t and t-1 are in three buffers.
 struct Apply128Lds {
-  double z[8 + 128 * 65];  // a staged half of U, row stride 65 (8 doubles of room in front)
-  double2 buf[2][128 + 12];
-  double2 pr[2][64];       // per wave: (c, s) of positions 64..127 for the current chain
-  double2 prl[2][64];      // per wave: (c, s) of positions 0..62 for the current chain
-  double ev[5][128];       // epilogue vectors
+  union {
+    double z[8 + 128 * 65];  // a staged half of U, row stride 65 (load and epilogue)
+    struct {
+      double2 buf[3][128 + 12];  // records of chains t+1, t, t-1
+      double2 pr[4][64];         // per wave: (c, s) of its group's positions
+      double xc[2][2][128];      // [step parity][0: column 63, 1: column 64] per row
+    } r;
+  };
+  double ev[5][128];    // epilogue vectors
+  double part[2][4][64];  // partial column sums of four row quarters
+  double hd[2][2][128];   // [0: x, 1: y][group] partial row products
 };
 
-__global__ __launch_bounds__(128) void k_tql_apply128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
-                                                      const double* __restrict__ mat,
-                                                      double* __restrict__ qmat,
-                                                      const double2* __restrict__ rlog,
-                                                      int64_t cap,
-                                                      const int32_t* __restrict__ status,
-                                                      const double* __restrict__ dvec, int nzc,
-                                                      double* __restrict__ vec,
-                                                      double* __restrict__ ampl,
-                                                      unsigned* __restrict__ err) {
+__global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+                                                       const double* __restrict__ mat,
+                                                       double* __restrict__ qmat,
+                                                       const double2* __restrict__ rlog,
+                                                       int64_t cap,
+                                                       const int32_t* __restrict__ status,
+                                                       const double* __restrict__ dvec, int nzc,
+                                                       double* __restrict__ vec,
+                                                       double* __restrict__ ampl,
+                                                       unsigned* __restrict__ err) {
   constexpr int NP = 128, H = 64, LD = 65;
   __shared__ Apply128Lds S;
   double* Z = S.z + 8;
   const int zl = blockIdx.x;
   const int64_t z = z0 + zl;
   const int tid = threadIdx.x;
+  const int g = tid >> 7, r = tid & 127, w = tid >> 6, lane = tid & 63;
   if (sel_ptr[z + 1] == sel_ptr[z]) {  // no observations: a = 0
-    vec[(size_t)zl * 4 * NP + NP + tid] = 0.0;
+    if (tid < NP) vec[(size_t)zl * 4 * NP + NP + tid] = 0.0;
     return;
   }
   const int stz = status[zl];
   if (stz & 3) return;  // failed zone: solved again by the fallback
   double* Q = qmat + (size_t)zl * NP * NP;
-  // upper half of the rows through LDS into registers, then the lower half
-  copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + H + (e & 63)]; },
-                   [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
+  double x[H];  // row r, columns 64 g .. 64 g + 63
+  copy_batched<16>(tid, NP * H, 256, [&](int e) { return Q[(e >> 6) * NP + H + (e & 63)]; },
+                   [&](int e, double v) { Z[(e >> 6) * LD + (e & 63)] = v; });
   __syncthreads();
-  double zr[H];
+  if (g == 1) {
 #pragma unroll
-  for (int c = 0; c < H; ++c) zr[c] = Z[tid * LD + c];
+    for (int c = 0; c < H; ++c) x[c] = Z[r * LD + c];
+  }
   __syncthreads();
-  copy_batched<16>(tid, NP * H, NP, [&](int e) { return Q[(e >> 6) * NP + (e & 63)]; },
-                   [&](int e, double x) { Z[(e >> 6) * LD + (e & 63)] = x; });
+  copy_batched<16>(tid, NP * H, 256, [&](int e) { return Q[(e >> 6) * NP + (e & 63)]; },
+                   [&](int e, double v) { Z[(e >> 6) * LD + (e & 63)] = v; });
   __syncthreads();
-  double zw[H];  // columns 0..63 (the compiler keeps part of them in AGPRs)
+  if (g == 0) {
 #pragma unroll
-  for (int c = 0; c < H; ++c) zw[c] = Z[tid * LD + c];
+    for (int c = 0; c < H; ++c) x[c] = Z[r * LD + c];
+  }
+  __syncthreads();  // Z is now reused by the record buffers
   const int64_t len = stz >> 2;
   const double2* lg = rlog + rlog_off(zl, cap);
-  if (tid < len) S.buf[0][tid] = lg[rlog_rec(tid)];
+  if (tid < len && tid < NP) S.r.buf[0][tid] = lg[rlog_rec(tid)];
   __syncthreads();
   int64_t pos = 0;
-  int b = 0;
-  while (pos < len) {
-    const double2 hd = S.buf[b][0];
-    const int m = __builtin_amdgcn_readfirstlane(((int)hd.x) >> 16);
-    const int cnt = __builtin_amdgcn_readfirstlane((int)hd.y);
-    const int l = m - cnt;
-    const int64_t nxt = pos + 1 + cnt;
-    const bool pf = nxt + tid < len;
+  int bt = 0, bp = 2, par = 0;
+  bool has_prev = false;
+  int m_prev = 0, l_prev = 0;
+  double2* pr = S.r.pr[w];
+  while (true) {
+    const bool has_cur = pos < len;
+    if (!has_cur && !has_prev) break;
+    int m = 0, l = 0;
+    int64_t nxt = pos;
+    if (has_cur) {
+      const double2 hd = S.r.buf[bt][0];
+      m = __builtin_amdgcn_readfirstlane(((int)hd.x) >> 16);
+      const int cnt = __builtin_amdgcn_readfirstlane((int)hd.y);
+      l = m - cnt;
+      nxt = pos + 1 + cnt;
+    }
+    const int bn = bt == 2 ? 0 : bt + 1;
+    const bool pf = has_cur && tid < NP && nxt + tid < len;
     double2 pv = make_double2(0.0, 0.0);
     if (pf) pv = lg[rlog_rec(nxt + tid)];
-    // record of position P: R(P) = buf[b][1 + (m - 1 - P)] = cr[2 (126 - P)], cr[2 (126 - P) + 1]
-    if (m - 1 >= H) {
-      // Positions 126 .. 64 as one unrolled sequence for every chain that
-      // reaches the register half: each wave writes the table pr[P] of its
-      // own, the chain's (c, s) for l <= P <= m-1 and the identity (1, 0)
-      // elsewhere (which leaves both columns unchanged exactly).  The LDS
-      // accesses of a wave are in order, so no barrier is needed.
-      const int lane = tid & 63;
-      double2* pr = S.pr[tid >> 6];
-      {
-        const int P = H + lane;
-        pr[lane] = (P >= l && P <= m - 1) ? S.buf[b][1 + (m - 1 - P)] : make_double2(1.0, 0.0);
-      }
-      // four segments of 16 positions (126..111, 110..95, 94..79, 78..64),
-      // each run only if the chain meets it; a segment's (c, s) are read at
-      // its start
-      auto segment = [&](auto top) {
-        constexpr int T = decltype(top)::value;
-        if (m - 1 < T - 15 || l > T) return;
-        double2 qv[16];
+    // coupling rotation (position 63) of chain t-1
+    if (has_prev && l_prev <= H - 1 && m_prev - 1 >= H - 1) {
+      const double2 q = S.r.buf[bp][1 + (m_prev - 1) - (H - 1)];
+      const double c64 = g == 1 ? x[0] : S.r.xc[par ^ 1][1][r];
+      const double c63 = g == 0 ? x[H - 1] : S.r.xc[par ^ 1][0][r];
+      const double n64 = fma(q.y, c63, q.x * c64);
+      const double n63 = fma(q.x, c63, -q.y * c64);
+      if (g == 1) x[0] = n64;
+      else x[H - 1] = n63;
+    }
+    if (g == 1) {
+      if (has_cur && m - 1 >= H) {
+        // chain t on columns 64..127: positions 126..64 in four unrolled
+        // segments of 16 with the identity (1, 0) outside l..m-1
+        {
+          const int P = H + lane;
+          pr[lane] = (P >= l && P <= m - 1) ? S.r.buf[bt][1 + (m - 1 - P)] : make_double2(1.0, 0.0);
+        }
+        auto segment = [&](auto top) {
+          constexpr int T = decltype(top)::value;
+          if (m - 1 < T - 15 || l > T) return;
+          double2 qv[16];
 #pragma unroll
-        for (int t = 0; t < 16; ++t)
-          if (T - t >= H) qv[t] = pr[T - t - H];
+          for (int t = 0; t < 16; ++t)
+            if (T - t >= H) qv[t] = pr[T - t - H];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
+          for (int t = 0; t < 16; ++t) {
             const int J = T - t;
-          if (J >= H) {
-            const double hh = zr[J + 1 - H], zj = zr[J - H];
-            zr[J + 1 - H] = fma(qv[t].y, zj, qv[t].x * hh);
-            zr[J - H] = fma(qv[t].x, zj, -qv[t].y * hh);
+            if (J >= H) {
+              const double hh = x[J + 1 - H], zj = x[J - H];
+              x[J + 1 - H] = fma(qv[t].y, zj, qv[t].x * hh);
+              x[J - H] = fma(qv[t].x, zj, -qv[t].y * hh);
+            }
           }
-        }
-      };
-      segment(std::integral_constant<int, 126>{});
-      segment(std::integral_constant<int, 110>{});
-      segment(std::integral_constant<int, 94>{});
-      segment(std::integral_constant<int, 78>{});
-    }
-    if (l < H) {
-      // position 63 (if the chain reaches it) couples zr[0] = column 64 with
-      // zw[63]; then positions 62 .. l as unrolled segments on zw, like the
-      // upper half (table prl of positions 0..62)
-      const int lane = tid & 63;
-      double2* prl = S.prl[tid >> 6];
-      prl[lane] = (lane >= l && lane <= m - 1 && lane <= H - 2) ? S.buf[b][1 + (m - 1 - lane)]
-                                                                 : make_double2(1.0, 0.0);
-      if (m - 1 >= H - 1) {
-        const double2 q = S.buf[b][1 + (m - 1) - (H - 1)];
-        const double hh = zr[0], zj = zw[H - 1];
-        zr[0] = fma(q.y, zj, q.x * hh);
-        zw[H - 1] = fma(q.x, zj, -q.y * hh);
+        };
+        segment(std::integral_constant<int, 126>{});
+        segment(std::integral_constant<int, 110>{});
+        segment(std::integral_constant<int, 94>{});
+        segment(std::integral_constant<int, 78>{});
       }
-      auto segment_lo = [&](auto top) {
-        constexpr int T = decltype(top)::value;
-        if (m - 1 < T - 15 || l > T) return;
-        double2 qv[16];
+      S.r.xc[par][1][r] = x[0];
+    } else {
+      if (has_prev && l_prev <= H - 2) {
+        // chain t-1 on columns 0..63: positions 62..l
+        const int mp = m_prev, lp = l_prev;
+        pr[lane] = (lane >= lp && lane <= mp - 1 && lane <= H - 2) ? S.r.buf[bp][1 + (mp - 1 - lane)]
+                                                                   : make_double2(1.0, 0.0);
+        auto segment_lo = [&](auto top) {
+          constexpr int T = decltype(top)::value;
+          if (mp - 1 < T - 15 || lp > T) return;
+          double2 qv[16];
 #pragma unroll
-        for (int t = 0; t < 16; ++t)
-          if (T - t >= 0) qv[t] = prl[T - t];
+          for (int t = 0; t < 16; ++t)
+            if (T - t >= 0) qv[t] = pr[T - t];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const int J = T - t;
-          if (J >= 0) {
-            const double hh = zw[J + 1], zj = zw[J];
-            zw[J + 1] = fma(qv[t].y, zj, qv[t].x * hh);
-            zw[J] = fma(qv[t].x, zj, -qv[t].y * hh);
+          for (int t = 0; t < 16; ++t) {
+            const int J = T - t;
+            if (J >= 0) {
+              const double hh = x[J + 1], zj = x[J];
+              x[J + 1] = fma(qv[t].y, zj, qv[t].x * hh);
+              x[J] = fma(qv[t].x, zj, -qv[t].y * hh);
+            }
           }
-        }
-      };
-      segment_lo(std::integral_constant<int, 62>{});
-      segment_lo(std::integral_constant<int, 46>{});
-      segment_lo(std::integral_constant<int, 30>{});
-      segment_lo(std::integral_constant<int, 14>{});
+        };
+        segment_lo(std::integral_constant<int, 62>{});
+        segment_lo(std::integral_constant<int, 46>{});
+        segment_lo(std::integral_constant<int, 30>{});
+        segment_lo(std::integral_constant<int, 14>{});
+      }
+      S.r.xc[par][0][r] = x[H - 1];
     }
-    if (pf) S.buf[b ^ 1][tid] = pv;
+    if (pf) S.r.buf[bn][tid] = pv;
     __syncthreads();
+    has_prev = has_cur;
+    m_prev = m;
+    l_prev = l;
+    bp = bt;
+    bt = bn;
     pos = nxt;
-    b ^= 1;
+    par ^= 1;
   }
-  // ---- epilogue: the quantities of k_eigvec_epi from the rows in hand ----
-  // Thread r holds row r of U in registers (zw, zr).  Column sums U^T w are
-  // taken from a half staged in LDS (column cc, rows 64 hf .. 64 hf + 63 per
-  // thread); row products U x from the registers.
+  // ---- epilogue: the quantities of k_eigvec_epi from the half rows in hand ----
   double* mu = S.ev[0];  // mu_j = max(lambda_j, 1) of column j
   double* bv = S.ev[1];  // b
   double* xv = S.ev[2];  // U^T b / mu, then U^T (b - M a0) / mu
   double* yv = S.ev[3];  // U^T 1_N sqrt(mu), then mu^-1/4
   double* av = S.ev[4];  // a0, then b - M a0, then vr^2
-  double* part = reinterpret_cast<double*>(S.buf);  // [4][64] partial column sums
   double* V = vec + (size_t)zl * 4 * NP;
-  mu[tid] = fmax(dvec[(size_t)tid * nzc + zl], 1.0);
-  bv[tid] = V[tid];
-  const int cc = tid & 63, hf = tid >> 6;
+  if (tid < NP) {
+    mu[tid] = fmax(dvec[(size_t)tid * nzc + zl], 1.0);
+    bv[tid] = V[tid];
+  }
+  const int cc = tid & 63, q4 = tid >> 6;
+  auto stage = [&](int grp) {
+    if (g == grp) {
 #pragma unroll
-  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zw[c];  // the lower half staged
-  __syncthreads();
-  // U^T b and U^T 1_N over the staged half (column offset co)
+      for (int c = 0; c < H; ++c) Z[r * LD + c] = x[c];
+    }
+    __syncthreads();
+  };
+  // U^T b and U^T 1_N over the staged half (column offset co), rows in quarters
   auto colsum2 = [&](int co) {
     double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
-#pragma unroll 4
-    for (int r0 = 64 * hf; r0 < 64 * hf + 64; r0 += 4) {
+#pragma unroll 2
+    for (int r0 = 32 * q4; r0 < 32 * q4 + 32; r0 += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int r = r0 + u;
-        const double zz = Z[r * LD + cc];
-        s1[u] = fma(zz, bv[r], s1[u]);
-        s2[u] += r < N ? zz : 0.0;
+        const int rr = r0 + u;
+        const double zz = Z[rr * LD + cc];
+        s1[u] = fma(zz, bv[rr], s1[u]);
+        s2[u] += rr < N ? zz : 0.0;
       }
     }
-    part[hf * 64 + cc] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
-    part[(2 + hf) * 64 + cc] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+    S.part[0][q4][cc] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    S.part[1][q4][cc] = (s2[0] + s2[1]) + (s2[2] + s2[3]);
     __syncthreads();
     if (tid < 64) {
-      const double c1 = part[tid] + part[64 + tid];
-      const double c2 = part[128 + tid] + part[192 + tid];
+      const double c1 = (S.part[0][0][tid] + S.part[0][1][tid]) + (S.part[0][2][tid] + S.part[0][3][tid]);
+      const double c2 = (S.part[1][0][tid] + S.part[1][1][tid]) + (S.part[1][2][tid] + S.part[1][3][tid]);
       xv[co + tid] = c1 / mu[co + tid];
       yv[co + tid] = c2 * sqrt(mu[co + tid]);
     }
     __syncthreads();
   };
-  // U^T w (w = av) over the staged half, divided by mu, into xv
   auto colsum1 = [&](int co) {
     double s1[4] = {0, 0, 0, 0};
-#pragma unroll 4
-    for (int r0 = 64 * hf; r0 < 64 * hf + 64; r0 += 4) {
+#pragma unroll 2
+    for (int r0 = 32 * q4; r0 < 32 * q4 + 32; r0 += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) s1[u] = fma(Z[(r0 + u) * LD + cc], av[r0 + u], s1[u]);
     }
-    part[hf * 64 + cc] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+    S.part[0][q4][cc] = (s1[0] + s1[1]) + (s1[2] + s1[3]);
     __syncthreads();
-    if (tid < 64) xv[co + tid] = (part[tid] + part[64 + tid]) / mu[co + tid];
+    if (tid < 64)
+      xv[co + tid] = ((S.part[0][0][tid] + S.part[0][1][tid]) + (S.part[0][2][tid] + S.part[0][3][tid])) /
+                     mu[co + tid];
     __syncthreads();
   };
-  // (U x)_r and (U y)_r in one pass: the lower half of row r from lo (LDS row
-  // or global row), the upper half from the registers; 16 entries at a time
-  auto rowdot2 = [&](const double* x, const double* y, double& tx, double& ty) {
-    tx = 0;
-    ty = 0;
+  // half-row products (U x)_r over the group's columns, into hd[.][g][r]
+  auto rowdot_h = [&](const double* xa, const double* ya) {
+    double tx = 0, ty = 0;
 #pragma unroll
-    for (int c0 = 0; c0 < NP; c0 += 16) {
-      asm volatile("" ::: "memory");
-      double2 uu[8], xx[8], yy[8];
+    for (int c0 = 0; c0 < H; c0 += 16) {
+      double2 xx[8], yy[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        if (c0 < H) uu[u] = make_double2(zw[c0 + 2 * u], zw[c0 + 2 * u + 1]);
-        else uu[u] = make_double2(zr[c0 - H + 2 * u], zr[c0 - H + 2 * u + 1]);
-        xx[u] = reinterpret_cast<const double2*>(x + c0)[u];
-        if (y) yy[u] = reinterpret_cast<const double2*>(y + c0)[u];
+        xx[u] = reinterpret_cast<const double2*>(xa + H * g + c0)[u];
+        if (ya) yy[u] = reinterpret_cast<const double2*>(ya + H * g + c0)[u];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        tx = fma(uu[u].x, xx[u].x, tx);
-        tx = fma(uu[u].y, xx[u].y, tx);
-        if (y) {
-          ty = fma(uu[u].x, yy[u].x, ty);
-          ty = fma(uu[u].y, yy[u].y, ty);
+        tx = fma(x[c0 + 2 * u], xx[u].x, tx);
+        tx = fma(x[c0 + 2 * u + 1], xx[u].y, tx);
+        if (ya) {
+          ty = fma(x[c0 + 2 * u], yy[u].x, ty);
+          ty = fma(x[c0 + 2 * u + 1], yy[u].y, ty);
         }
       }
     }
+    S.hd[0][g][r] = tx;
+    if (ya) S.hd[1][g][r] = ty;
   };
-  colsum2(0);  // the lower half is in LDS
-#pragma unroll
-  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zr[c];
+  stage(0);
+  colsum2(0);
+  stage(1);
+  colsum2(H);  // the upper half stays staged
+  rowdot_h(xv, yv);
   __syncthreads();
-  colsum2(H);  // upper half
-  double a0, vr;
-  rowdot2(xv, yv, a0, vr);
-  av[tid] = a0;
-  __syncthreads();
-  // refinement a = a0 + U diag(1/mu) U^T (b - M a0); M symmetric: column tid
-  const double* M = mat + (size_t)zl * NP * NP;
-  double mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int c0 = 0; c0 < NP; c0 += 64) {  // 64 loads in flight (the HBM latency once per half)
-    double mv[64];
-#pragma unroll
-    for (int u = 0; u < 64; ++u) mv[u] = M[(size_t)(c0 + u) * NP + tid];
-#pragma unroll
-    for (int u = 0; u < 64; ++u) mq[u & 7] = fma(mv[u], av[c0 + u], mq[u & 7]);
+  double a0 = 0, vr = 0;
+  if (tid < NP) {
+    a0 = S.hd[0][0][tid] + S.hd[0][1][tid];
+    vr = S.hd[1][0][tid] + S.hd[1][1][tid];
+    av[tid] = a0;
   }
-  const double ma = ((mq[0] + mq[1]) + (mq[2] + mq[3])) + ((mq[4] + mq[5]) + (mq[6] + mq[7]));
   __syncthreads();
-  av[tid] = bv[tid] - ma;
+  // refinement a = a0 + U diag(1/mu) U^T (b - M a0); M symmetric: row r of
+  // M a0 from column r, half of the columns per group (32 loads in flight)
+  {
+    const double* M = mat + (size_t)zl * NP * NP;
+    double mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c0 = 0; c0 < H; c0 += 32) {
+      double mv[32];
+#pragma unroll
+      for (int u = 0; u < 32; ++u) mv[u] = M[(size_t)(H * g + c0 + u) * NP + r];
+#pragma unroll
+      for (int u = 0; u < 32; ++u) mq[u & 7] = fma(mv[u], av[H * g + c0 + u], mq[u & 7]);
+    }
+    S.hd[0][g][r] = ((mq[0] + mq[1]) + (mq[2] + mq[3])) + ((mq[4] + mq[5]) + (mq[6] + mq[7]));
+  }
+  __syncthreads();
+  if (tid < NP) av[tid] = bv[tid] - (S.hd[0][0][tid] + S.hd[0][1][tid]);
   __syncthreads();
   colsum1(H);  // upper half still staged
-#pragma unroll
-  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zw[c];
-  __syncthreads();
+  stage(0);
   colsum1(0);
-  double ux, unused;
-  rowdot2(xv, nullptr, ux, unused);
-  const double ar = a0 + ux;
-  av[tid] = vr * vr;
-  yv[tid] = 1.0 / sqrt(sqrt(mu[tid]));
+  rowdot_h(xv, nullptr);
   __syncthreads();
-  double nq[4] = {0, 0, 0, 0};
-#pragma unroll 8
-  for (int q = 0; q < NP; q += 4) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) nq[u] += av[q + u];
+  double ar = 0;
+  if (tid < NP) {
+    ar = a0 + (S.hd[0][0][tid] + S.hd[0][1][tid]);
+    av[tid] = vr * vr;
+    yv[tid] = 1.0 / sqrt(sqrt(mu[tid]));
   }
-  const double nv = (nq[0] + nq[1]) + (nq[2] + nq[3]);
-  const double vn = vr / sqrt(nv);
-  V[NP + tid] = ar;
-  V[2 * NP + tid] = vn;
-  V[3 * NP + tid] = mu[tid];
-  if (ampl && tid < N) ampl[(size_t)z * N + tid] = ar;
-  if (ar != ar) atomicOr(err, OAKG_DERR_NAN);
-  // Y = U diag(mu^-1/4): the lower half from LDS, then the upper half through LDS
-  for (int e = tid; e < NP * H; e += NP)
+  __syncthreads();
+  if (tid < NP) {
+    double nq[4] = {0, 0, 0, 0};
+#pragma unroll 8
+    for (int q = 0; q < NP; q += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) nq[u] += av[q + u];
+    }
+    const double nv = (nq[0] + nq[1]) + (nq[2] + nq[3]);
+    const double vn = vr / sqrt(nv);
+    V[NP + tid] = ar;
+    V[2 * NP + tid] = vn;
+    V[3 * NP + tid] = mu[tid];
+    if (ampl && tid < N) ampl[(size_t)z * N + tid] = ar;
+    if (ar != ar) atomicOr(err, OAKG_DERR_NAN);
+  }
+  // Y = U diag(mu^-1/4): the lower half is staged, then the upper half
+  for (int e = tid; e < NP * H; e += 256)
     Q[(e >> 6) * NP + (e & 63)] = Z[(e >> 6) * LD + (e & 63)] * yv[e & 63];
   __syncthreads();
+  if (g == 1) {
 #pragma unroll
-  for (int c = 0; c < H; ++c) Z[tid * LD + c] = zr[c] * yv[H + c];
+    for (int c = 0; c < H; ++c) Z[r * LD + c] = x[c] * yv[H + c];
+  }
   __syncthreads();
-  for (int e = tid; e < NP * H; e += NP) Q[(e >> 6) * NP + H + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
+  for (int e = tid; e < NP * H; e += 256) Q[(e >> 6) * NP + H + (e & 63)] = Z[(e >> 6) * LD + (e & 63)];
 }
 
 // ---------------------------------------------------------------------------
@@ -1221,7 +1268,7 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0,
                      A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
   if constexpr (NP == 128) {  // the epilogue runs at the end of the apply kernel
-    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(128), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
+    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
                        A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,
                        A.vec, A.ampl, A.err);
   } else {

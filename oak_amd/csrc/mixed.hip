@@ -17,7 +17,7 @@
 //                      spread (T ~ I); with |C| up to ~1e5 (precise SSH
 //                      observations) that is ~1e-3 of the spread.  Rounding
 //                      S itself perturbs those directions only at second order.
-//   k_update_f32       (N <= 64; analysis128.hip's k_xform128 for NP = 128)
+//   k_update_f32       (N <= 64; analysis128.hip's k_xform128f for NP = 128)
 //                      Ea = xa + sqrt(N-1) infl (Sf T) on v_mfma_f32_16x16x4f32
 //                      with T rounded to FP32; xa = xbar + clamp(Sf a) in FP64
 //                      (the FP32 sum of N terms Sf_k a_k cancels strongly)

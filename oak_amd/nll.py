@@ -164,9 +164,9 @@ def tune(prob, log_sigma0, L0, infl0, steps=50, lr=0.05, ctx=None, callback=None
         J.backward()
         hist.append(float(J.detach()))
         if callback:
-            callback(it, float(J), ls, lL, la)
+            callback(it, float(J.detach()), ls, lL, la)
         opt.step()
     with torch.no_grad():
         J = innovation_nll(ls, lL, la, prob, ctx)
     hist.append(float(J))
-    return ls.detach(), float(torch.exp(lL)), float(torch.exp(la)), hist
+    return ls.detach(), float(torch.exp(lL.detach())), float(torch.exp(la.detach())), hist

@@ -69,3 +69,36 @@ def test_context_creation_fails_loudly_without_gpu():
     from oak_amd import Context, OakError
     with pytest.raises(OakError):
         Context(0)
+
+
+def test_header_compiles_as_c99(tmp_path):
+    """include/oakg.h is plain C: a C99 translation unit that takes the
+    address of every entry point compiles without warnings."""
+    import subprocess
+    src = tmp_path / "use.c"
+    names = sorted(_declared())
+    body = "\n".join("  (void)&%s;" % n for n in names)
+    src.write_text('#include "oakg.h"\nint main(void) {\n%s\n  return 0;\n}\n' % body)
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-pedantic", "-c",
+                        "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(tmp_path / "use.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_report_log_lines_use_fortran_formats():
+    """write_report mirrors report's '(A,A,F15.4)' and '(A,A,E15.7)' lines
+    (assimilation.F90:3606-3620)."""
+    import io
+    from oak_amd.assimilation import _fortran_e, write_report
+    assert _fortran_e(1.5) == "  0.1500000E+01"
+    assert _fortran_e(-0.000123456789) == " -0.1234568E-03"
+    assert _fortran_e(9.99999996) == "  0.1000000E+02"  # rounding carries into the exponent
+    assert _fortran_e(0.0) == "  0.0000000E+00"
+    assert _fortran_e(float("nan")).strip() == "NaN"
+    buf = io.StringIO()
+    write_report(buf, "007.forecast.", 55000.5, [3.0, 1.5, -0.2, 0.3, 2e-5])
+    lines = buf.getvalue().splitlines()
+    assert lines[0] == "007.forecast.mjd                           55000.5000"
+    assert lines[1] == "007.forecast.rms_yo-Hx                  0.1500000E+01"
+    assert lines[2] == "007.forecast.bias_yo-Hx                -0.2000000E+00"
+    assert lines[4] == "007.forecast.projection_yo-Hx_into_HSf  0.2000000E-04"

@@ -395,3 +395,22 @@ def test_error_paths(oak):
     y = d["y"].copy(); y[0] = np.nan
     with pytest.raises(NaNError):
         R.analysis(d["xf"], Hxf, y, d["Sf"], HSf, R.DiagCovar(d["sig2"]))
+
+
+def test_golden_fixture(oak):
+    """The committed small ensemble case (tests/golden/ens_small.npz):
+    the analysed ensemble and both reports against the stored oracle
+    outputs."""
+    _, A = oak
+    d = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "ens_small.npz")))
+    H = A.coo_to_csr(len(d["yo"]), d["Hi"], d["Hj"], d["Hs"])
+    geo = A.Geometry(d["zone_ptr"], d["zpos"], d["L"], d["cutoff"], d["ox"], d["oy"])
+    Ea = A.ensemble_analysis(geo, H, d["Hshift"], d["yo"], d["rmse2"], d["E"],
+                             excluded=d["excluded"], inflation=float(d["inflation"]),
+                             max_correction=float(d["max_correction"]))
+    xf = d["E"].mean(axis=1)[:, None]
+    assert rel_err(Ea - xf, d["Ea"] - xf) < TOL64
+    rf = A.ensemble_report(H, d["Hshift"], d["yo"], d["rmse2"], d["E"], d["excluded"])
+    ra = A.ensemble_report(H, d["Hshift"], d["yo"], d["rmse2"], Ea, d["excluded"])
+    np.testing.assert_allclose(rf[0], d["report_forecast"], rtol=1e-11)
+    np.testing.assert_allclose(ra[0], d["report_analysis"], rtol=1e-9)

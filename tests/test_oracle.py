@@ -390,3 +390,19 @@ def test_ensemble_report_against_sample_covariance():
         exp = [g, math.sqrt((dd[a:b][v] ** 2).sum() / g), dd[a:b][v].sum() / g,
                math.sqrt((dd[a:b] * r).sum() / g), math.sqrt(r @ P @ r / g)]
         np.testing.assert_allclose(out[k], exp, rtol=1e-11)
+
+
+# --- the committed golden fixture (tests/golden/make_golden.py) ---
+def test_oracle_reproduces_golden_fixture():
+    """The oracle still gives the committed expected outputs of the small
+    ensemble case (inflation, a binding maxCorrection, two excluded
+    observations, forecast and analysis reports)."""
+    import os
+    from tests.golden.make_golden import expected
+    d = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "ens_small.npz")))
+    got = expected(d)
+    xf = d["E"].mean(axis=1)[:, None]
+    err = np.abs((got["Ea"] - xf) - (d["Ea"] - xf)).max() / np.abs(d["Ea"] - xf).max()
+    assert err < 1e-10, err
+    np.testing.assert_allclose(got["report_forecast"], d["report_forecast"], rtol=1e-12)
+    np.testing.assert_allclose(got["report_analysis"], d["report_analysis"], rtol=1e-10)

@@ -42,78 +42,51 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
   double bacc[8];
 #pragma unroll
   for (int J = 0; J < 8; ++J) bacc[J] = 0;
-  // Software pipeline over the steps of 4 observations: the selection entry
-  // (index, weight) is loaded two steps ahead and the observation's row of
-  // S, r and r * innov one step ahead, so the gathers of the next step are in
-  // flight while this step's MFMAs run (two steps ahead measured the same).
   const int64_t nsteps = (p + 3) >> 2;
-  struct Ent {
-    int64_t l;
-    double w;
-    bool ok;
-  };
-  struct Row {
-    double r, rd, bv[8];
-  };
-  auto load_ent = [&](int64_t st) {
-    Ent e{0, 0.0, false};
-    const int64_t q = 4 * st + kq;
-    if (q < p) {
-      e.l = sel_idx[p0 + q];
-      e.w = sel_w[p0 + q];
-      e.ok = true;
-    }
-    return e;
-  };
-  auto load_row = [&](Ent e) {
-    Row o;
-    o.r = 0;
-    o.rd = 0;
-    int64_t l = e.l;
-    bool ok = e.ok;
-    if (ok && (l < 0 || l >= m)) {
-      atomicOr(err, OAKG_DERR_IDX);
-      ok = false;
-    }
-    if (!ok) l = 0;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const int64_t q = 4 * s + kq;
+    double r = 0, rd = 0;
+    int64_t l = 0;
+    bool ok = q < p;
     if (ok) {
-      o.r = (e.w * e.w) * rho[l];
-      o.rd = o.r * innov[l];
+      l = sel_idx[p0 + q];
+      if (l < 0 || l >= m) {
+        atomicOr(err, OAKG_DERR_IDX);
+        ok = false;
+        l = 0;
+      } else {
+        const double w = sel_w[p0 + q];
+        r = (w * w) * rho[l];
+        rd = r * innov[l];
+      }
     }
     // columns 16 J + jl, J = 0..7, stored contiguously (hs_col<128>)
+    double bv[8];
     const TS* row = HS + (size_t)l * P128 + 8 * jl;
     if constexpr (sizeof(TS) == 4) {
       const float4 u0 = reinterpret_cast<const float4*>(row)[0];
       const float4 u1 = reinterpret_cast<const float4*>(row)[1];
-      o.bv[0] = u0.x; o.bv[1] = u0.y; o.bv[2] = u0.z; o.bv[3] = u0.w;
-      o.bv[4] = u1.x; o.bv[5] = u1.y; o.bv[6] = u1.z; o.bv[7] = u1.w;
+      bv[0] = u0.x; bv[1] = u0.y; bv[2] = u0.z; bv[3] = u0.w;
+      bv[4] = u1.x; bv[5] = u1.y; bv[6] = u1.z; bv[7] = u1.w;
     } else {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
         const double2 u = reinterpret_cast<const double2*>(row)[h];
-        o.bv[2 * h] = u.x;
-        o.bv[2 * h + 1] = u.y;
+        bv[2 * h] = u.x;
+        bv[2 * h + 1] = u.y;
       }
     }
 #pragma unroll
-    for (int J = 0; J < 8; ++J) o.bv[J] = ok ? o.bv[J] : 0.0;
-    return o;
-  };
-  Ent e1 = load_ent(1);
-  Row cur = load_row(load_ent(0));
-  for (int64_t s = 0; s < nsteps; ++s) {
-    const Row nxt = load_row(e1);  // step s + 1 (zero beyond the end)
-    e1 = load_ent(s + 2);
-    const double aA = cur.r * cur.bv[IA], aB = cur.r * cur.bv[IB];
+    for (int J = 0; J < 8; ++J) bv[J] = ok ? bv[J] : 0.0;
+    const double aA = r * bv[IA], aB = r * bv[IB];
 #pragma unroll
-    for (int J = 0; J <= IA; ++J) accA[J] = mfma_f64(aA, cur.bv[J], accA[J]);
+    for (int J = 0; J <= IA; ++J) accA[J] = mfma_f64(aA, bv[J], accA[J]);
 #pragma unroll
-    for (int J = 0; J <= IB; ++J) accB[J] = mfma_f64(aB, cur.bv[J], accB[J]);
+    for (int J = 0; J <= IB; ++J) accB[J] = mfma_f64(aB, bv[J], accB[J]);
     if (W == 0) {
 #pragma unroll
-      for (int J = 0; J < 8; ++J) bacc[J] = fma(cur.rd, cur.bv[J], bacc[J]);
+      for (int J = 0; J < 8; ++J) bacc[J] = fma(rd, bv[J], bacc[J]);
     }
-    cur = nxt;
   }
   auto store = [&](int I, int J, const d4& acc) {
 #pragma unroll

@@ -185,3 +185,31 @@ def test_forecast_and_analysis_reports(A):
     _close(rf, O.ensemble_report(coo, P.Hshift, P.yo, P.rmse2, P.E), 1e-11)
     _close(ra, O.ensemble_report(coo, P.Hshift, P.yo, P.rmse2, Ea), 1e-11)
     assert ra[0, 1] < rf[0, 1] and ra[0, 3] < rf[0, 3]
+
+
+def test_ensemble_report_device_tensors_and_ranges(A):
+    """FP64 and FP32 device ensembles with three observation ranges: the
+    same statistics as the host call (FP64: the same bits) and the oracle."""
+    import torch
+    P, ex = _ens_problem(100)
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    cut = [0, 150, 300, P.m]
+    host = A.ensemble_report(H, P.Hshift, P.yo, P.rmse2, P.E, ex, ranges=cut)
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt)
+    Hd = (t(H[0], torch.int64), t(H[1], torch.int32), t(H[2]))
+    Ed = t(P.E.T).t()
+    got = A.ensemble_report(Hd, t(P.Hshift), t(P.yo), t(P.rmse2), Ed, t(ex, torch.uint8),
+                            ranges=cut)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got.cpu().numpy(), host)
+    exp = O.ensemble_report((P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
+                            1.0 - ex.astype(float), ranges=cut)
+    _close(host, exp, 1e-11)
+    E32 = t(P.E.astype(np.float32).T, torch.float32).t()
+    got32 = A.ensemble_report(Hd, t(P.Hshift), t(P.yo), t(P.rmse2), E32, t(ex, torch.uint8),
+                              ranges=cut).cpu().numpy()
+    exp32 = O.ensemble_report((P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2,
+                              P.E.astype(np.float32).astype(np.float64), 1.0 - ex.astype(float),
+                              ranges=cut)
+    _close(got32, exp32, 1e-5)

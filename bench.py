@@ -62,33 +62,50 @@ def device_inputs(DP, device):
 
 def flops_model(N, counts, zone_sizes):
     """Algorithmic FLOP per analysis, per phase, from the per-zone local
-    observation counts p_z (zones with p_z > 0):
+    observation counts p_z (zones with p_z > 0), counted as the kernels
+    compute them:
       gram   2 N^2 p + 2 N p          (S^T R^-1 S and S^T R^-1 d)
-      eig    9 N^3                    (symmetric eigensolve with vectors)
-      trans  6 N^3                    (Omega, U^T Omega, U (.))
-      update 2 s N^2                  (Sf_z T)"""
+      eig    9 N^3                    (symmetric eigensolve with vectors,
+                                       Golub & Van Loan's count for
+                                       tridiagonalisation + QL)
+      trans  N^2 (N + 1) + 10 N^2     (T = Y Y^T Omega: the symmetric
+                                       product Y Y^T and Omega applied as its
+                                       rank-structured correction; OAK's dense
+                                       form, RotateVector + three N x N GEMMs,
+                                       would be 6 N^3)
+      update 2 s N^2 + 2 s N          (Sf_z [T | a])"""
     act = counts > 0
     p = counts[act].astype(np.float64)
     s = zone_sizes[act].astype(np.float64)
+    nact = float(act.sum())
     return {
         "gram": float(np.sum(2.0 * N * N * p + 2.0 * N * p)),
-        "eig": float(9.0 * N ** 3 * act.sum()),
-        "trans": float(6.0 * N ** 3 * act.sum()),
-        "update": float(np.sum(2.0 * s * N * N)),
+        "eig": 9.0 * N ** 3 * nact,
+        "trans": (N * N * (N + 1.0) + 10.0 * N * N) * nact,
+        "update": float(np.sum(2.0 * s * N * N + 2.0 * s * N)),
     }
 
 
-def cpu_baseline(DP, target_s=15.0):
+def cpu_baseline(DP, target_s=15.0, gpu_results=None):
     """Oracle (LAPACK backend, OpenMP dynamic over zones, OAK's brute-force
     selection) on a strided sample of zones spread over the whole domain (every
     k-th zone, so that dense and sparse observation regions are represented in
-    proportion); the sampled zones' state rows are gathered into one slice."""
+    proportion); the sampled zones' state rows are gathered into one slice.
+
+    gpu_results: {label: Ea (device tensor, n x N)} of the GPU analyses of the
+    same inputs (inflation 1, no clamp): their rows of the sampled zones are
+    compared with the oracle's analysis, Ea = xa + sqrt(N-1) Sa
+    (assimilation.F90:3318-3320), by oak_amd.errors.ensemble_errors.
+    Returns (baseline dict, {label: errors})."""
     import torch
+    from oak_amd.errors import ensemble_errors
     from oracle import oracle as O
     P = DP.host
     E = DP.E
     N = E.shape[1]
     lapack = O.load_lapack()
+    # the host cores of one GPU's share of the box (16 per GPU); OMP_NUM_THREADS
+    # overrides
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     # obs-space anomalies for every observation (computed on the GPU: this
     # leg times the zone loop of locAnalysis, not H·E)
@@ -116,21 +133,34 @@ def cpu_baseline(DP, target_s=15.0):
                    cutoff=np.ascontiguousarray(P.max_len[sel]), metric=P.metric,
                    loctype=P.loctype, ox=P.ox, oy=P.oy)
         t0 = time.perf_counter()
-        O.loc_analysis_geo(geo, xf, Hxf, P.yo, P.rmse2, Sf, HSf, nthreads=cores)
-        return time.perf_counter() - t0, len(rows_), len(sel)
+        xa, Sa, _ = O.loc_analysis_geo(geo, xf, Hxf, P.yo, P.rmse2, Sf, HSf, nthreads=cores)
+        return time.perf_counter() - t0, len(rows_), len(sel), (rows_, Ez, xa, Sa)
 
     probe_stride = max(1, nz // max(4 * cores, 64))
-    t, ne, nzs = run(probe_stride)
+    t, ne, nzs, _ = run(probe_stride)
     per_zone = t / nzs
     print("# cpu baseline probe: %d zones in %.2f s" % (nzs, t), file=sys.stderr, flush=True)
     want = max(4 * cores, int(target_s / max(per_zone, 1e-6)))
     stride = max(1, nz // want)
-    t, ne, nzs = run(stride)
-    return {"value": ne / t, "unit": "analysed state elems/s", "cores": cores, "kind": "port",
+    t, ne, nzs, (rows_, Ez, xa, Sa) = run(stride)
+    base = {"value": ne / t, "unit": "analysed state elems/s", "cores": cores, "kind": "port",
             "sample": "%d zones (every %d-th of %d; %d state elements) of the same workload, "
                       "zone loop of locAnalysis with OAK's brute-force selection; %s "
                       "eigensolver, %.1f s" % (nzs, stride, nz, ne,
                                                "LAPACK dsyev (OpenBLAS)" if lapack else "Jacobi", t)}
+    errs = {}
+    if gpu_results:
+        Eo = xa[:, None] + np.sqrt(N - 1.0) * Sa
+        del Sa
+        ri = torch.as_tensor(rows_, device=E.device)
+        for label, Ea in gpu_results.items():
+            Eg = Ea[ri, :].double().cpu().numpy()
+            errs[label] = dict(ensemble_errors(Eg, Eo, Ez), zones=int(nzs),
+                               elements=int(ne))
+            del Eg
+            print("# %s vs oracle on the sample: %s" % (label, errs[label]), file=sys.stderr,
+                  flush=True)
+    return base, errs
 
 
 PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
@@ -203,6 +233,26 @@ def run_report(args, H, Hshift, yo, rmse2, Ef, Ea):
         file=sys.stderr, flush=True)
     return {"ms_per_call": ms, "forecast": dict(zip(REPORT_FIELDS, rf.tolist())),
             "analysis": dict(zip(REPORT_FIELDS, ra.tolist()))}
+
+
+def phase_rooflines(F, phase, mixed):
+    """Achieved rate of every phase against the MI355X peak of the arithmetic
+    it runs on: the Gram (FP64 MFMA accumulation in both paths), the
+    eigensolve (FP64), the transform and update (FP32 MFMA on the mixed path,
+    FP64 otherwise)."""
+    work = {"gram": F["gram"], "eig": F["eig"], "tmat": F["trans"], "update": F["update"]}
+    if phase["tmat"] == 0.0:  # N > 64: transform and update in one kernel
+        work["update"] += work["tmat"]
+        del work["tmat"]
+    out = {}
+    for k, w in work.items():
+        if phase[k] <= 0:
+            continue
+        peak = PEAK_FP32_MFMA if (mixed and k in ("update", "tmat")) else PEAK_FP64
+        ach = w / (phase[k] * 1e-3)
+        out[k] = {"ms": round(phase[k], 3), "flop": w, "achieved": ach / 1e12,
+                  "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak}
+    return out
 
 
 def roofline(F, phase, mixed, eig_name, config, N):
@@ -415,11 +465,13 @@ def main():
                    "max_p": int(counts.max()),
                    "phase_ms": {k: round(v, 3) for k, v in phase.items()}},
         "roofline": roofline(F, phase, head == "mixed", eig_name, args.config, N),
+        "roofline_phases": phase_rooflines(F, phase, head == "mixed"),
     }
     if "mixed" in res and "fp64" in res:
         ms64, ph64, _ = res["fp64"]
         out["fp64"] = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64,
-                       "phase_ms": {k: round(v, 3) for k, v in ph64.items()}}
+                       "phase_ms": {k: round(v, 3) for k, v in ph64.items()},
+                       "roofline_phases": phase_rooflines(F, ph64, False)}
         out["mixed_speedup_vs_fp64"] = ms64 / ms
         e = ensemble_errors(Ea32, Ea64, DP.E)
         out["mixed_err_vs_fp64"] = dict(e, bound=1e-4)
@@ -434,7 +486,16 @@ def main():
         print("# calib copy: 1073741824 bytes read, 1073741824 bytes written", file=sys.stderr,
               flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(DP)
+        gpu_res = {}
+        if Ea64 is not None:
+            gpu_res["fp64"] = Ea64
+        if "mixed" in res:
+            gpu_res["mixed"] = Ea32
+        out["cpu_baseline"], errs = cpu_baseline(DP, gpu_results=gpu_res)
+        # the analyses against the FP64 oracle on the sampled zones (bounds:
+        # 1e-10 for the FP64 path, 1e-4 for the mixed path)
+        out["oracle_err"] = {k: dict(v, bound=1e-10 if k == "fp64" else 1e-4)
+                             for k, v in errs.items()}
     print(json.dumps(out), flush=True)
 
 

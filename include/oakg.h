@@ -179,11 +179,16 @@ int oakg_select_observations(oakg_ctx* ctx, const oakg_zones* zones, const oakg_
 /* oak_assim: Ef (n x N ensemble, zone-permuted) -> Ea.  H·Ef + Hshift gives the
  * observed ensemble; anomalies are scaled by 1/sqrt(N-1) (ppdef.h:52);
  * Sa is multiplied by inflation (inflation.mult) and xa - xf is clamped to
- * +-max_correction (Model.maxCorrection; pass HUGE_VAL for none). */
+ * +-maxCorrection (assimilation.F90:3301-3312).  OAK's maxCorrection is a
+ * per-element array, filled per variable from Model.maxCorrection(v) and
+ * permuted like the state (assimilation.F90:462-476): max_correction_elem
+ * gives it ([n], zone-permuted, every entry >= 0), or NULL to clamp every
+ * element to the scalar max_correction (HUGE_VAL for none, OAK's default). */
 int oakg_ensemble_analysis(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs,
                            const oakg_csr* H, const double* Hshift, int32_t N,
                            const double* Ef, int64_t ldEf, double* Ea, int64_t ldEa,
-                           double inflation, double max_correction, uint32_t flags);
+                           double inflation, double max_correction,
+                           const double* max_correction_elem, uint32_t flags);
 
 /* Mixed-precision ensemble analysis: FP32 ensembles, the contractions
  * S = H·A', S^T R^-1 S and A'·W in FP32/BF16 with FP32 accumulation, the
@@ -192,7 +197,8 @@ int oakg_ensemble_analysis(oakg_ctx* ctx, const oakg_zones* zones, const oakg_ob
 int oakg_ensemble_analysis_f32(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs,
                                const oakg_csr* H, const float* Hshift, int32_t N,
                                const float* Ef, int64_t ldEf, float* Ea, int64_t ldEa,
-                               double inflation, double max_correction, uint32_t flags);
+                               double inflation, double max_correction,
+                               const double* max_correction_elem, uint32_t flags);
 
 /* Innovation statistics of report (assimilation.F90:3584-3628), which Assim
  * writes to the log for the forecast and the analysis (:3389-3391) and for

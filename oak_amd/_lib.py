@@ -112,7 +112,7 @@ def load():
         C.POINTER(C.c_int64), C.c_uint32]
     L.oakg_ensemble_analysis.argtypes = [
         _vp, C.POINTER(Zones), C.POINTER(Obs), C.POINTER(Csr), _vp, C.c_int32, _vp, C.c_int64,
-        _vp, C.c_int64, C.c_double, C.c_double, C.c_uint32]
+        _vp, C.c_int64, C.c_double, C.c_double, _vp, C.c_uint32]
     L.oakg_ensemble_analysis_f32.argtypes = L.oakg_ensemble_analysis.argtypes
     L.oakg_local_nll.argtypes = [
         _vp, C.POINTER(Zones), C.POINTER(Obs), C.c_int32, _vp, _vp, C.c_int64, C.c_double, _vp,
@@ -156,10 +156,13 @@ class Context:
     def handle(self):
         return self._h
 
+    def last_error(self):
+        return load().oakg_last_error(self._h).decode(errors="replace")
+
     def check(self, rc):
         if rc == OAKG_OK:
             return
-        msg = load().oakg_last_error(self._h).decode(errors="replace")
+        msg = self.last_error()
         if rc == OAKG_ENAN:
             raise NaNError(rc, msg)
         raise OakError(rc, msg)

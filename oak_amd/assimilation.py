@@ -201,7 +201,11 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
     With torch device tensors, ``out`` may give the preallocated Ea.
     A float32 ensemble (numpy or torch) runs the mixed-precision path
     (oakg_ensemble_analysis_f32: FP32 state and MFMA contractions, FP64
-    eigensolve); float64 runs the FP64 path."""
+    eigensolve); float64 runs the FP64 path.
+
+    max_correction is OAK's maxCorrection (assimilation.F90:462-476,
+    3308-3312): a scalar for every element, or one value per state element
+    (n, zone-permuted; float64, a device tensor with device inputs)."""
     ctx = ctx or default_context()
     L = _lib.load()
     dev = _is_dev(Ef)
@@ -209,7 +213,18 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
     rp, co, va = H
     m = len(yo)
     n, N = Ef.shape
-    maxc = float(max_correction) if np.isfinite(max_correction) else float("inf")
+    maxc_elem = None
+    if np.ndim(max_correction) == 0:
+        maxc = float(max_correction) if np.isfinite(max_correction) else float("inf")
+    else:
+        maxc = float("inf")
+        maxc_elem = max_correction
+        if len(maxc_elem) != n:
+            raise ValueError("max_correction: one value per state element (n = %d)" % n)
+        if not dev:
+            maxc_elem = np.ascontiguousarray(maxc_elem, dtype=np.float64)
+        elif maxc_elem.dtype != __import__("torch").float64 or not maxc_elem.is_contiguous():
+            raise TypeError("max_correction: a contiguous float64 device tensor")
     if dev:
         import torch
         ldE = _check_dev_matrix(Ef, n, N, "Ef")
@@ -226,7 +241,7 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
             raise TypeError("Ef must be float64 or float32")
         rc = fn(ctx.handle, C.byref(z), C.byref(o), C.byref(h), Hshift.data_ptr(), N,
                 Ef.data_ptr(), ldE, Ea.data_ptr(), ldA, float(inflation), maxc,
-                _lib.OAKG_DEVICE_PTRS)
+                _lib.ptr(maxc_elem), _lib.OAKG_DEVICE_PTRS)
         ctx.check(rc)
         return Ea
     mixed = np.asarray(Ef).dtype == np.float32
@@ -241,7 +256,8 @@ def ensemble_analysis(geo, H, Hshift, yo, rmse2, Ef, excluded=None, inflation=1.
     Ea = np.zeros((n, N), order="F", dtype=sdt)
     fn = L.oakg_ensemble_analysis_f32 if mixed else L.oakg_ensemble_analysis
     rc = fn(ctx.handle, C.byref(z), C.byref(o), C.byref(h), Hshift.ctypes.data, N,
-            Ef.ctypes.data, n, Ea.ctypes.data, n, float(inflation), maxc, 0)
+            Ef.ctypes.data, n, Ea.ctypes.data, n, float(inflation), maxc,
+            _lib.ptr(maxc_elem), 0)
     ctx.check(rc)
     return Ea
 

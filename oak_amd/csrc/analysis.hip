@@ -538,7 +538,8 @@ __global__ __launch_bounds__(256) void k_update(int64_t z0, int N, const int64_t
                                                 const double* __restrict__ S, int64_t ldS,  // Sf or E
                                                 double* __restrict__ xa, double* __restrict__ Out,
                                                 int64_t ldO, double scaling, double infl,
-                                                double maxcorr) {
+                                                double maxcorr,
+                                                const double* __restrict__ maxcorr_v) {
 #pragma clang fp contract(off)
   constexpr int NB = NP / 16;
   constexpr int LDW = NP + 17;
@@ -583,8 +584,9 @@ __global__ __launch_bounds__(256) void k_update(int64_t z0, int N, const int64_t
       const double x0 = X0[i0 + row];
       xar = x0 + dx;
       if (MODE == 1) {
-        if (xar - maxcorr > x0) xar = x0 + maxcorr;
-        if (xar < x0 - maxcorr) xar = x0 - maxcorr;
+        const double mc = maxcorr_v ? maxcorr_v[i0 + row] : maxcorr;
+        if (xar - mc > x0) xar = x0 + mc;
+        if (xar < x0 - mc) xar = x0 - mc;
       }
       if (MODE == 0 && kk == 0) xa[i0 + row] = xar;
     }
@@ -692,11 +694,11 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   else if (A.mode == 0)
     hipLaunchKernelGGL((k_update<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr,
                        A.mat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl,
-                       A.maxcorr);
+                       A.maxcorr, A.maxcorr_v);
   else
     hipLaunchKernelGGL((k_update<NP, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr,
                        A.mat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl,
-                       A.maxcorr);
+                       A.maxcorr, A.maxcorr_v);
   if (tm) tm->mark(PH_END);
 }
 

@@ -99,6 +99,7 @@ struct ChunkArgs {
   double* Out;  // Sa (mode 0) or Ea (mode 1)
   int64_t ldO;
   double scaling, infl, maxcorr;
+  const double* maxcorr_v;  // per-element clamp [n] (zone-permuted) or NULL: maxcorr for all
   // mixed precision: 1 = FP32 S (HS32), FP32 members (S32) and FP32 Out32
   int prec;
   const float* HS32;

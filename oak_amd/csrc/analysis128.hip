@@ -486,7 +486,8 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
                                                   const TE* __restrict__ S, int64_t ldS,
                                                   double* __restrict__ xa, TE* __restrict__ Out,
                                                   int64_t ldO, double scaling, double infl,
-                                                  double maxcorr) {
+                                                  double maxcorr,
+                                                  const double* __restrict__ maxcorr_v) {
 #pragma clang fp contract(off)
   // FP64 state, update on FP64 MFMA (MODE 0: locAnalysis with Sf given,
   // MODE 1: ensemble); FP32 members run in k_xform128f.
@@ -669,8 +670,9 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
     dxp += shfl_xor_d(dxp, 32);
     double xar = x0 + dxp;
     if (MODE == 1) {
-      if (xar - maxcorr > x0) xar = x0 + maxcorr;
-      if (xar < x0 - maxcorr) xar = x0 - maxcorr;
+      const double mc = (maxcorr_v && rok) ? maxcorr_v[i0 + row] : maxcorr;
+      if (xar - mc > x0) xar = x0 + mc;
+      if (xar < x0 - mc) xar = x0 - mc;
     }
     if (MODE == 0 && rok && kk == 0) xa[i0 + row] = xar;
     // two passes over the output tiles (4 each) keep the accumulators small
@@ -740,7 +742,8 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, int N, const int6
                                                    const double* __restrict__ X0,
                                                    const float* __restrict__ S, int64_t ldS,
                                                    float* __restrict__ Out, int64_t ldO,
-                                                   double scaling, double infl, double maxcorr) {
+                                                   double scaling, double infl, double maxcorr,
+                                                   const double* __restrict__ maxcorr_v) {
 #pragma clang fp contract(off)
   __shared__ XformLdsF L;
   const int zl = blockIdx.x;
@@ -903,8 +906,9 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, int N, const int6
     dxp += shfl_xor_d(dxp, 16);
     dxp += shfl_xor_d(dxp, 32);
     double xar = x0 + dxp;
-    if (xar - maxcorr > x0) xar = x0 + maxcorr;
-    if (xar < x0 - maxcorr) xar = x0 - maxcorr;
+    const double mc = (maxcorr_v && rok) ? maxcorr_v[i0 + row] : maxcorr;
+    if (xar - mc > x0) xar = x0 + mc;
+    if (xar < x0 - mc) xar = x0 - mc;
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
       const float* xh = xr + 64 * h * LDF;
@@ -969,15 +973,15 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (A.prec == 1)
     hipLaunchKernelGGL(k_xform128f, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S32, A.ldS, A.Out32, A.ldO, A.scaling,
-                       A.infl, A.maxcorr);
+                       A.infl, A.maxcorr, A.maxcorr_v);
   else if (A.mode == 0)
     hipLaunchKernelGGL((k_xform128<0, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
-                       A.infl, A.maxcorr);
+                       A.infl, A.maxcorr, A.maxcorr_v);
   else
     hipLaunchKernelGGL((k_xform128<1, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
-                       A.infl, A.maxcorr);
+                       A.infl, A.maxcorr, A.maxcorr_v);
   if (tm) tm->mark(PH_END);
 }
 

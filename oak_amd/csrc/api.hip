@@ -20,6 +20,25 @@
 #define OAKG_PIPE_SLOTS 3  // zone chunks in flight on as many streams
 #endif
 
+namespace oakg {
+// Bump allocator over a fixed device buffer (the no-allocation entry point's
+// scratch array): buffers of one call, 256-byte aligned, released together.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* get(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > cap || bytes > cap - at) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+  size_t left() const {
+    const size_t at = (used + 255) & ~(size_t)255;
+    return at < cap ? cap - at : 0;
+  }
+};
+}  // namespace oakg
+
 struct oakg_ctx {
   int device = 0;
   int precision = OAKG_FP64;
@@ -43,6 +62,11 @@ struct oakg_ctx {
   // internal streams of the chunk pipeline (default priority) and their events
   hipStream_t aux[OAKG_PIPE_SLOTS - 1] = {};
   hipEvent_t ev_fork = nullptr, ev_join[OAKG_PIPE_SLOTS - 1] = {};
+  // events around each call's device work (its ms_total), created once
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  // set only during a call of oakg_local_analysis_csr_noalloc: every buffer
+  // of the call comes from this arena (the device's scratch array)
+  oakg::Arena* arena = nullptr;
 };
 
 namespace oakg {
@@ -138,6 +162,13 @@ void arg_fail(oakg_ctx* c, const std::string& msg) {
   throw Fail{OAKG_EARG};
 }
 
+// A device buffer of the call: from the context's grow-only workspace, or,
+// during a no-allocation call, from the scratch arena (nullptr when full).
+void* wsget(oakg_ctx* c, Workspace::Slot s, size_t bytes) {
+  if (c->arena) return c->arena->get(bytes == 0 ? 8 : bytes);
+  return c->ws.get(s, bytes);
+}
+
 int padded_n(int N) {
   if (N <= 16) return 16;
   if (N <= 32) return 32;
@@ -151,7 +182,7 @@ template <typename T>
 const T* stage(oakg_ctx* c, Workspace::Slot s, const T* p, int64_t count, bool dev) {
   if (!p || count <= 0) return p;
   if (dev) return p;
-  T* d = (T*)c->ws.get(s, sizeof(T) * count);
+  T* d = (T*)wsget(c, s, sizeof(T) * count);
   if (!d) check(c, hipErrorOutOfMemory, "staging");
   check(c, hipMemcpyAsync(d, p, sizeof(T) * count, hipMemcpyHostToDevice, c->stream), "H2D");
   return d;
@@ -166,7 +197,7 @@ const T* stage2d(oakg_ctx* c, Workspace::Slot s, const T* p, int64_t rows, int c
     *ld_out = ld;
     return p;
   }
-  T* d = (T*)c->ws.get(s, sizeof(T) * std::max<int64_t>(rows, 1) * cols);
+  T* d = (T*)wsget(c, s, sizeof(T) * std::max<int64_t>(rows, 1) * cols);
   if (!d) check(c, hipErrorOutOfMemory, "staging");
   if (rows > 0 && ld == rows)
     check(c, hipMemcpyAsync(d, p, sizeof(T) * rows * cols, hipMemcpyHostToDevice, c->stream),
@@ -262,9 +293,16 @@ void validate_obs(oakg_ctx* c, const oakg_obs* o, bool need_geo, int loctype) {
 void begin_call(oakg_ctx* c) {
   c->err.clear();
   check(c, hipSetDevice(c->device), "hipSetDevice");
-  if (!c->d_err) {
-    check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
-    check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
+  if (c->arena) {
+    // no-allocation call: the error word and the statistics come first in
+    // the scratch (NoallocCall swaps them in); no events are created
+  } else {
+    if (!c->d_err) {
+      check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
+      check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
+    }
+    if (!c->ev_t0) check(c, hipEventCreate(&c->ev_t0), "event");
+    if (!c->ev_t1) check(c, hipEventCreate(&c->ev_t1), "event");
   }
   fill_u32(c->d_err, 2, 0u, c->stream);  // [0] error bits, [1] eigensolver fallbacks
   memset(&c->stats, 0, sizeof c->stats);
@@ -345,25 +383,36 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
   const size_t budget = c->ws_budget;
-  const int nslot = (!c->timing && nz >= 4096) ? OAKG_PIPE_SLOTS : 1;
+  const int nslot = (!c->timing && !c->arena && nz >= 4096) ? OAKG_PIPE_SLOTS : 1;
   int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
   zc = std::min<int64_t>(zc, 1 << 20);
   if (nslot > 1) zc = std::min<int64_t>(zc, (nz + nslot - 1) / nslot);  // a chunk per slot
   zc = std::min<int64_t>(zc, nz);
   zc = (zc + 63) & ~(int64_t)63;  // slots start on 64-zone groups of the rotation log
+  if (c->arena) {
+    // chunks of what is left of the scratch (7 buffers of up to 255 bytes of
+    // alignment), in whole 64-zone groups of the rotation log
+    const size_t left = c->arena->left();
+    const int64_t fit = left > 8 * 256 ? (int64_t)((left - 8 * 256) / per_zone) : 0;
+    zc = std::min<int64_t>(zc, fit & ~(int64_t)63);
+    if (zc < 64) {
+      c->err = "the no-allocation scratch is too small for one group of 64 zones at this N";
+      throw Fail{OAKG_EARG};
+    }
+  }
   const int64_t zs = zc * nslot;
-  double* mat = (double*)c->ws.get(Workspace::MAT, sizeof(double) * (size_t)NP * NP * zs);
-  double* ymat = (double*)c->ws.get(Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zs);
-  double* vec = (double*)c->ws.get(Workspace::VEC, sizeof(double) * (size_t)4 * NP * zs);
+  double* mat = (double*)wsget(c, Workspace::MAT, sizeof(double) * (size_t)NP * NP * zs);
+  double* ymat = (double*)wsget(c, Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zs);
+  double* vec = (double*)wsget(c, Workspace::VEC, sizeof(double) * (size_t)4 * NP * zs);
   if (!mat || !ymat || !vec) check(c, hipErrorOutOfMemory, "workspace");
   double *dvec = nullptr, *evec = nullptr;
   char* rlog = nullptr;
   int32_t* tst = nullptr;
   if (ql) {
-    dvec = (double*)c->ws.get(Workspace::DVEC, sizeof(double) * (size_t)NP * zs);
-    evec = (double*)c->ws.get(Workspace::EVEC, sizeof(double) * (size_t)NP * zs);
-    rlog = (char*)c->ws.get(Workspace::RLOG, (size_t)16 * cap * zs);
-    tst = (int32_t*)c->ws.get(Workspace::TQL_STATUS, sizeof(int32_t) * zs);
+    dvec = (double*)wsget(c, Workspace::DVEC, sizeof(double) * (size_t)NP * zs);
+    evec = (double*)wsget(c, Workspace::EVEC, sizeof(double) * (size_t)NP * zs);
+    rlog = (char*)wsget(c, Workspace::RLOG, (size_t)16 * cap * zs);
+    tst = (int32_t*)wsget(c, Workspace::TQL_STATUS, sizeof(int32_t) * zs);
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
     fill_i32(sweeps_all, nz, 0, c->stream);
   }
@@ -456,10 +505,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   if (ldSf < n || ldSa < n || (m > 0 && ldHSf < m)) arg_fail(c, "leading dimension too small");
   begin_call(c);
   if (c->timing) c->timer.mark(PH_PREP);
-  hipEvent_t e0, e1;
-  check(c, hipEventCreate(&e0), "event");
-  check(c, hipEventCreate(&e1), "event");
-  check(c, hipEventRecord(e0, c->stream), "event");
+  if (!c->arena) check(c, hipEventRecord(c->ev_t0, c->stream), "event");
 
   oakg_zones zz{};
   if (csr) {
@@ -475,9 +521,9 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   const double* dxf = stage(c, Workspace::IN_XF, xf, n, dev);
   const double* dHxf = stage(c, Workspace::IN_HXF, Hxf, m, dev);
 
-  double* HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
-  double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
-  double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+  double* HS = (double*)wsget(c, Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+  double* innov = (double*)wsget(c, Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+  double* rho = (double*)wsget(c, Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
   if (!HS || !innov || !rho) check(c, hipErrorOutOfMemory, "workspace");
   PrepLocArgs P{m, N, dHSf, ld_hs, dHxf, g.od.yo, g.od.rmse2, g.od.excluded, HS, innov, rho,
                 c->d_err, 1.0};
@@ -517,15 +563,15 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
 
   double* d_ampl = nullptr;
   if (ampl) {
-    d_ampl = dev ? ampl : (double*)c->ws.get(Workspace::OUT_AMPL, sizeof(double) * (size_t)N * std::max<int64_t>(nz, 1));
+    d_ampl = dev ? ampl : (double*)wsget(c, Workspace::OUT_AMPL, sizeof(double) * (size_t)N * std::max<int64_t>(nz, 1));
     if (!d_ampl) check(c, hipErrorOutOfMemory, "workspace");
     fill_f64(d_ampl, (int64_t)N * nz, 0.0, c->stream);
   }
-  double* dxa = dev ? xa : (double*)c->ws.get(Workspace::OUT_XA, sizeof(double) * std::max<int64_t>(n, 1));
-  double* dSa = dev ? Sa : (double*)c->ws.get(Workspace::OUT_S, sizeof(double) * std::max<int64_t>(n, 1) * N);
+  double* dxa = dev ? xa : (double*)wsget(c, Workspace::OUT_XA, sizeof(double) * std::max<int64_t>(n, 1));
+  double* dSa = dev ? Sa : (double*)wsget(c, Workspace::OUT_S, sizeof(double) * std::max<int64_t>(n, 1) * N);
   const int64_t ld_sa = dev ? ldSa : n;
   if (!dxa || !dSa) check(c, hipErrorOutOfMemory, "workspace");
-  int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+  int32_t* sweeps = (int32_t*)wsget(c, Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
   if (!sweeps) check(c, hipErrorOutOfMemory, "workspace");
 
   ChunkArgs A{};
@@ -554,12 +600,10 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   A.maxcorr = HUGE_VAL;
   run_zones(c, NP, A, nz, sweeps);
   if (c->timing) c->timer.mark(PH_END);
-  check(c, hipEventRecord(e1, c->stream), "event");
+  if (!c->arena) check(c, hipEventRecord(c->ev_t1, c->stream), "event");
   finish_call(c, nz, sel.ptr, sweeps);
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!c->arena) (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
   collect_stats(c, nz, sel.ptr, ms);
   (void)total;
   if (!dev) {
@@ -597,20 +641,17 @@ int local_nll_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, in
   if (!dev) validate_zone_ptr_host(c, nz, zones->zone_ptr);
   begin_call(c);
   if (c->timing) c->timer.mark(PH_PREP);
-  hipEvent_t e0, e1;
-  check(c, hipEventCreate(&e0), "event");
-  check(c, hipEventCreate(&e1), "event");
-  check(c, hipEventRecord(e0, c->stream), "event");
+  check(c, hipEventRecord(c->ev_t0, c->stream), "event");
   Geo g = stage_geo(c, zones, obs, dev, true);
   int64_t ld_hs = 0;
   const double* dHSf = stage2d(c, Workspace::IN_HSF, HSf, m, N, ldHSf, dev, &ld_hs);
   const double* dHxf = stage(c, Workspace::IN_HXF, Hxf, m, dev);
-  double* HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
-  double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
-  double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
-  double* zbuf = (double*)c->ws.get(Workspace::NLL_Z, sizeof(double) * (3 * std::max<int64_t>(nz, 1) + 2));
+  double* HS = (double*)wsget(c, Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+  double* innov = (double*)wsget(c, Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+  double* rho = (double*)wsget(c, Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+  double* zbuf = (double*)wsget(c, Workspace::NLL_Z, sizeof(double) * (3 * std::max<int64_t>(nz, 1) + 2));
   double* gth = dev ? grad_log_sigma
-                    : (double*)c->ws.get(Workspace::NLL_GTH, sizeof(double) * std::max<int64_t>(m, 1));
+                    : (double*)wsget(c, Workspace::NLL_GTH, sizeof(double) * std::max<int64_t>(m, 1));
   if (!HS || !innov || !rho || !zbuf || !gth) check(c, hipErrorOutOfMemory, "workspace");
   double* Jz = zbuf;
   double* gLz = zbuf + nz;
@@ -625,7 +666,7 @@ int local_nll_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, in
   SelDev sel{};
   int64_t total = 0;
   check(c, run_selection(c->ws, g.zd, g.od, 1, c->stream, c->d_err, sel, &total), "selection");
-  int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+  int32_t* sweeps = (int32_t*)wsget(c, Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
   if (!sweeps) check(c, hipErrorOutOfMemory, "workspace");
   ChunkArgs A{};
   A.N = N;
@@ -651,12 +692,10 @@ int local_nll_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, in
   launch_nll_totals(nz, Jz, gAz, tot, c->stream);
   check(c, hipGetLastError(), "totals");
   if (c->timing) c->timer.mark(PH_END);
-  check(c, hipEventRecord(e1, c->stream), "event");
+  check(c, hipEventRecord(c->ev_t1, c->stream), "event");
   finish_call(c, nz, sel.ptr, sweeps);
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
   collect_stats(c, nz, sel.ptr, ms);
   const hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   check(c, hipMemcpy(nll, tot, sizeof(double), k), "copy nll");
@@ -674,7 +713,8 @@ int local_nll_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, in
 template <typename T>
 int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, const oakg_csr* H,
                   const T* Hshift, int32_t N, const T* Ef, int64_t ldEf, T* Ea, int64_t ldEa,
-                  double inflation, double max_correction, uint32_t flags) {
+                  double inflation, double max_correction, const double* maxc_elem,
+                  uint32_t flags) {
   constexpr bool mixed = std::is_same<T, float>::value;
   return guarded(c, [&] {
     const bool dev = flags & OAKG_DEVICE_PTRS;
@@ -703,13 +743,14 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
     }
     if (H->n != n) arg_fail(c, "H->n must equal the state size");
     if (ldEf < n || ldEa < n) arg_fail(c, "leading dimension too small");
-    if (!(inflation > 0) || !(max_correction > 0)) arg_fail(c, "inflation and max_correction must be > 0");
+    if (!(inflation > 0) || !(max_correction >= 0))
+      arg_fail(c, "inflation must be > 0 and max_correction >= 0");
+    if (maxc_elem && !dev)
+      for (int64_t i = 0; i < n; ++i)
+        if (!(maxc_elem[i] >= 0)) arg_fail(c, "max_correction_elem: every entry must be >= 0");
     begin_call(c);
     if (c->timing) c->timer.mark(PH_PREP);
-    hipEvent_t e0, e1;
-    check(c, hipEventCreate(&e0), "event");
-    check(c, hipEventCreate(&e1), "event");
-    check(c, hipEventRecord(e0, c->stream), "event");
+    check(c, hipEventRecord(c->ev_t0, c->stream), "event");
     Geo g = stage_geo(c, zones, obs, dev, true);
     int64_t ld_e = 0;
     const T* dE = stage2d(c, Workspace::IN_S, Ef, n, N, ldEf, dev, &ld_e);
@@ -717,16 +758,17 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
     const int32_t* dcol = stage(c, Workspace::H_COL, H->col, H->nnz, dev);
     const double* dval = stage(c, Workspace::H_VAL, H->val, H->nnz, dev);
     const T* dHs = stage(c, Workspace::H_SHIFT, Hshift, m, dev);
-    double* xbar = (double*)c->ws.get(Workspace::XBAR, sizeof(double) * std::max<int64_t>(n, 1));
-    double* HEc = (double*)c->ws.get(Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
+    const double* dmc = stage(c, Workspace::MAXCORR, maxc_elem, n, dev);
+    double* xbar = (double*)wsget(c, Workspace::XBAR, sizeof(double) * std::max<int64_t>(n, 1));
+    double* HEc = (double*)wsget(c, Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
     double* HS = nullptr;
     float* HS32 = nullptr;
     if (mixed)
-      HS32 = (float*)c->ws.get(Workspace::HS32, sizeof(float) * std::max<int64_t>(m, 1) * NP);
+      HS32 = (float*)wsget(c, Workspace::HS32, sizeof(float) * std::max<int64_t>(m, 1) * NP);
     else
-      HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
-    double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
-    double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+      HS = (double*)wsget(c, Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+    double* innov = (double*)wsget(c, Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+    double* rho = (double*)wsget(c, Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
     if (!xbar || !HEc || !(mixed ? (void*)HS32 : (void*)HS) || !innov || !rho)
       check(c, hipErrorOutOfMemory, "workspace");
     const double scaling = sqrt((double)N - 1.0);
@@ -762,9 +804,9 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
     SelDev sel{};
     int64_t total = 0;
     check(c, run_selection(c->ws, g.zd, g.od, 1, c->stream, c->d_err, sel, &total), "selection");
-    T* dEa = dev ? Ea : (T*)c->ws.get(Workspace::OUT_S, sizeof(T) * std::max<int64_t>(n, 1) * N);
+    T* dEa = dev ? Ea : (T*)wsget(c, Workspace::OUT_S, sizeof(T) * std::max<int64_t>(n, 1) * N);
     const int64_t ld_ea = dev ? ldEa : n;
-    int32_t* sweeps = (int32_t*)c->ws.get(Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
+    int32_t* sweeps = (int32_t*)wsget(c, Workspace::SWEEPS, sizeof(int32_t) * std::max<int64_t>(nz, 1));
     if (!dEa || !sweeps) check(c, hipErrorOutOfMemory, "workspace");
     ChunkArgs A{};
     A.N = N;
@@ -797,14 +839,13 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
     A.scaling = scaling;
     A.infl = inflation;
     A.maxcorr = max_correction;
+    A.maxcorr_v = dmc;
     run_zones(c, NP, A, nz, sweeps);
     if (c->timing) c->timer.mark(PH_END);
-    check(c, hipEventRecord(e1, c->stream), "event");
+    check(c, hipEventRecord(c->ev_t1, c->stream), "event");
     finish_call(c, nz, sel.ptr, sweeps);
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
     collect_stats(c, nz, sel.ptr, ms);
     if (!dev) copy_out_2d(c, Ea, ldEa, (const T*)dEa, n, N, "D2H Ea");
     return OAKG_OK;
@@ -825,11 +866,11 @@ void run_report(oakg_ctx* c, int NP, ReportArgs A, int32_t nranges, const int64_
   }
   pbeg[nranges] = (int64_t)pcs.size();
   const int64_t np = (int64_t)pcs.size();
-  RepPiece* dp = (RepPiece*)c->ws.get(Workspace::REP_PIECES, sizeof(RepPiece) * std::max<int64_t>(np, 1));
-  int64_t* db = (int64_t*)c->ws.get(Workspace::REP_PBEG, sizeof(int64_t) * (nranges + 1));
-  double* part = (double*)c->ws.get(Workspace::REP_PART,
+  RepPiece* dp = (RepPiece*)wsget(c, Workspace::REP_PIECES, sizeof(RepPiece) * std::max<int64_t>(np, 1));
+  int64_t* db = (int64_t*)wsget(c, Workspace::REP_PBEG, sizeof(int64_t) * (nranges + 1));
+  double* part = (double*)wsget(c, Workspace::REP_PART,
                                     sizeof(double) * (4 + NP) * std::max<int64_t>(np, 1));
-  double* dst = dev ? stats : (double*)c->ws.get(Workspace::REP_STATS, sizeof(double) * 5 * nranges);
+  double* dst = dev ? stats : (double*)wsget(c, Workspace::REP_STATS, sizeof(double) * 5 * nranges);
   if (!dp || !db || !part || !dst) check(c, hipErrorOutOfMemory, "workspace");
   if (np > 0)
     check(c, hipMemcpyAsync(dp, pcs.data(), sizeof(RepPiece) * np, hipMemcpyHostToDevice, c->stream),
@@ -937,15 +978,15 @@ int ensemble_report_impl(oakg_ctx* c, const oakg_obs* obs, const oakg_csr* H, co
     const int32_t* dcol = stage(c, Workspace::H_COL, H->col, H->nnz, dev);
     const double* dval = stage(c, Workspace::H_VAL, H->val, H->nnz, dev);
     const T* dHs = stage(c, Workspace::H_SHIFT, Hshift, m, dev);
-    double* HEc = (double*)c->ws.get(Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
+    double* HEc = (double*)wsget(c, Workspace::HEC, sizeof(double) * std::max<int64_t>(m, 1) * N);
     double* HS = nullptr;
     float* HS32 = nullptr;
     if (mixed)
-      HS32 = (float*)c->ws.get(Workspace::HS32, sizeof(float) * std::max<int64_t>(m, 1) * NP);
+      HS32 = (float*)wsget(c, Workspace::HS32, sizeof(float) * std::max<int64_t>(m, 1) * NP);
     else
-      HS = (double*)c->ws.get(Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
-    double* innov = (double*)c->ws.get(Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
-    double* rho = (double*)c->ws.get(Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
+      HS = (double*)wsget(c, Workspace::HS, sizeof(double) * std::max<int64_t>(m, 1) * NP);
+    double* innov = (double*)wsget(c, Workspace::INNOV, sizeof(double) * std::max<int64_t>(m, 1));
+    double* rho = (double*)wsget(c, Workspace::RHO, sizeof(double) * std::max<int64_t>(m, 1));
     if (!HEc || !(mixed ? (void*)HS32 : (void*)HS) || !innov || !rho)
       check(c, hipErrorOutOfMemory, "workspace");
     PrepEnsArgs P{};
@@ -1035,6 +1076,8 @@ int oakg_destroy(oakg_ctx* c) {
     if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
+  if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
   if (c->d_err) (void)hipFree(c->d_err);
   if (c->d_stats) (void)hipFree(c->d_stats);
   c->ws.release();
@@ -1136,17 +1179,19 @@ int oakg_select_observations(oakg_ctx* c, const oakg_zones* zones, const oakg_ob
 int oakg_ensemble_analysis(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs,
                            const oakg_csr* H, const double* Hshift, int32_t N, const double* Ef,
                            int64_t ldEf, double* Ea, int64_t ldEa, double inflation,
-                           double max_correction, uint32_t flags) {
+                           double max_correction, const double* max_correction_elem,
+                           uint32_t flags) {
   return ensemble_impl<double>(c, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, inflation,
-                               max_correction, flags);
+                               max_correction, max_correction_elem, flags);
 }
 
 int oakg_ensemble_analysis_f32(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs,
                                const oakg_csr* H, const float* Hshift, int32_t N, const float* Ef,
                                int64_t ldEf, float* Ea, int64_t ldEa, double inflation,
-                               double max_correction, uint32_t flags) {
+                               double max_correction, const double* max_correction_elem,
+                               uint32_t flags) {
   return ensemble_impl<float>(c, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, inflation,
-                              max_correction, flags);
+                              max_correction, max_correction_elem, flags);
 }
 
 int oakg_report(oakg_ctx* c, int64_t m, int32_t N, const double* yo_Hx, const double* HS,

@@ -6,14 +6,12 @@
 #include <string>
 #include <vector>
 
+#include "select_cells.hpp"
+
 namespace oakg {
 
 constexpr int kMaxCells = 1 << 22;  // cap of the observation bin grid
 
-struct GridInfo {
-  double x0, y0, h;
-  int gx, gy;
-};
 
 // Device views of the call's inputs (all device pointers).
 struct ZonesDev {
@@ -50,7 +48,7 @@ class Workspace {
     // staged copies of host inputs and outputs
     Z_PTR, Z_POS, Z_CORR, Z_MAXLEN, O_X, O_Y, O_Z, O_T, O_YO, O_RMSE2, O_EXCL, IN_S, IN_HSF,
     IN_XF, IN_HXF, H_RP, H_COL, H_VAL, H_SHIFT, OUT_XA, OUT_S, OUT_AMPL, U_SEL_PTR, U_SEL_IDX,
-    U_SEL_W, REP_PIECES, REP_PBEG, REP_PART, REP_STATS, REP_D, REP_HS, NSLOTS
+    U_SEL_W, MAXCORR, REP_PIECES, REP_PBEG, REP_PART, REP_STATS, REP_D, REP_HS, NSLOTS
   };
   Workspace() : ptr_(NSLOTS, nullptr), cap_(NSLOTS, 0) {}
   ~Workspace() { release(); }

@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void k_update_f32(int64_t z0, int N,
                                                     const double* __restrict__ xbar,
                                                     const float* __restrict__ E, int64_t ldE,
                                                     float* __restrict__ Out, int64_t ldO,
-                                                    double scaling, double infl, double maxcorr) {
+                                                    double scaling, double infl, double maxcorr,
+                                                    const double* __restrict__ maxcorr_v) {
   constexpr int NB = NP / 16;
   constexpr int LDW = NP + 17;  // the 4 k-rows of one MFMA step land on distinct banks
   __shared__ float Ws[NP * LDW];
@@ -159,8 +160,9 @@ __global__ __launch_bounds__(256) void k_update_f32(int64_t z0, int N,
     dxp += __shfl_xor(dxp, 32, 64);
     if (rok) {
       double xar = xbd + dxp;
-      if (xar - maxcorr > xbd) xar = xbd + maxcorr;
-      if (xar < xbd - maxcorr) xar = xbd - maxcorr;
+      const double mc = maxcorr_v ? maxcorr_v[i0 + row] : maxcorr;
+      if (xar - mc > xbd) xar = xbd + mc;
+      if (xar < xbd - mc) xar = xbd - mc;
       float* Orow = Out + (i0 + row);
 #pragma unroll
       for (int J = 0; J < NB; ++J) {
@@ -201,7 +203,8 @@ void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st) {
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st) {
 #define OAK_UPD32(P)                                                                             \
   hipLaunchKernelGGL(k_update_f32<P>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr, A.mat, \
-                     A.vec, A.X0, A.S32, A.ldS, A.Out32, A.ldO, A.scaling, A.infl, A.maxcorr)
+                     A.vec, A.X0, A.S32, A.ldS, A.Out32, A.ldO, A.scaling, A.infl, A.maxcorr, \
+                     A.maxcorr_v)
   switch (NP) {
     case 16: OAK_UPD32(16); break;
     case 32: OAK_UPD32(32); break;

@@ -102,22 +102,6 @@ __global__ void k_gather_sorted(int64_t m, const int32_t* __restrict__ sidx,
   sy[p] = oy[l];
 }
 
-// Cell-index range a zone's cutoff disc can touch, with a margin that covers
-// the rounding of the bin arithmetic (exact selection is decided by the
-// distance test itself).
-__device__ __forceinline__ void zone_cells(const GridInfo& g, double zx, double zy, double c,
-                                           int& cx0, int& cx1, int& cy0, int& cy1) {
-  double a0 = (zx - c - g.x0) / g.h, a1 = (zx + c - g.x0) / g.h;
-  double b0 = (zy - c - g.y0) / g.h, b1 = (zy + c - g.y0) / g.h;
-  double eps = 1e-7;
-  cx0 = (int)fmax(floor(a0 - eps), -1.0);
-  cx1 = (int)fmin(floor(a1 + eps), (double)g.gx);
-  cy0 = (int)fmax(floor(b0 - eps), -1.0);
-  cy1 = (int)fmin(floor(b1 + eps), (double)g.gy);
-  cx0 = max(cx0, 0); cy0 = max(cy0, 0);
-  cx1 = min(cx1, g.gx - 1); cy1 = min(cy1, g.gy - 1);
-}
-
 // One wave per zone.  FILL = false: count; FILL = true: write the list.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_select_binned(
@@ -134,6 +118,7 @@ __global__ __launch_bounds__(256) void k_select_binned(
   double zx = zpos[4 * z], zy = zpos[4 * z + 1];
   double c = maxlen[z], L = corr[z];
   int cx0, cx1, cy0, cy1;
+  // a zone whose disc misses the bin grid gets the empty range (cy0 > cy1)
   zone_cells(g, zx, zy, c, cx0, cx1, cy0, cy1);
   int64_t cnt = 0;
   int64_t base = FILL ? sel_ptr[z] : 0;

@@ -88,13 +88,13 @@ module oakg
     end function
 
     integer(c_int) function oakg_ensemble_analysis(ctx, zones, obs, H, Hshift, N, Ef, ldEf, Ea, ldEa, &
-         inflation, max_correction, flags) bind(C, name="oakg_ensemble_analysis")
+         inflation, max_correction, max_correction_elem, flags) bind(C, name="oakg_ensemble_analysis")
       import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, oakg_zones, oakg_obs, oakg_csr
       type(c_ptr), value :: ctx
       type(oakg_zones), intent(in) :: zones
       type(oakg_obs), intent(in) :: obs
       type(oakg_csr), intent(in) :: H
-      type(c_ptr), value :: Hshift, Ef, Ea
+      type(c_ptr), value :: Hshift, Ef, Ea, max_correction_elem
       integer(c_int32_t), value :: N
       integer(c_int64_t), value :: ldEf, ldEa
       real(c_double), value :: inflation, max_correction

@@ -1,12 +1,16 @@
 ! Fortran driver of locAnalysis_gpu: a small 1-D local analysis with a
-! Gaussian taper callback (OAK's selectObservations interface).  Writes the
-! inputs, the outputs and the return code to a binary file for the tests.
+! Gaussian taper callback (OAK's selectObservations interface, called with the
+! state index of a zone's first element).  Runs it with localise_obs = .true.
+! and .false. and writes the inputs, the outputs and the return codes to a
+! binary file for the tests.
 !
 !   oakg_fdriver <output file>
 module fdriver_geom
   implicit none
   integer, parameter :: nzone = 10, zsize = 2, nmem = 6, nobs = 8
 contains
+  ! i: 1-based state index (OAK passes startIndex(zi), rrsqrt.F90:368);
+  ! element i sits at x = 0.5 i + 3, observation l at x = 1.25 l
   subroutine select_gauss(i, c, relevantObs)
     integer, intent(in) :: i
     real(8), intent(out) :: c(:)
@@ -14,8 +18,8 @@ contains
     integer :: l
     real(8) :: d
     do l = 1, size(c)
-      d = abs(dble(i) - 1.25d0 * dble(l))
-      relevantObs(l) = d <= 5.0d0
+      d = abs(0.5d0 * dble(i) + 3.0d0 - 1.25d0 * dble(l))
+      relevantObs(l) = d <= 2.0d0
       c(l) = exp(-(d / 2.0d0)**2)
     end do
   end subroutine select_gauss
@@ -26,8 +30,9 @@ program oakg_fdriver
   use oakg_rrsqrt
   use fdriver_geom
   implicit none
-  integer :: zoneSize(nzone), i, k, l, info, u, n
+  integer :: zoneSize(nzone), i, k, l, info, info2, u, n
   real(c_double), allocatable, target :: xf(:), Hxf(:), yo(:), Sf(:,:), HSf(:,:), Rd(:), xa(:), Sa(:,:)
+  real(c_double), allocatable, target :: xa2(:), Sa2(:,:)
   character(len=1024) :: fname
   logical, allocatable :: excl(:)
 
@@ -35,6 +40,7 @@ program oakg_fdriver
   n = nzone * zsize
   zoneSize = zsize
   allocate(xf(n), Hxf(nobs), yo(nobs), Sf(n, nmem), HSf(nobs, nmem), Rd(nobs), xa(n), Sa(n, nmem))
+  allocate(xa2(n), Sa2(n, nmem))
   do i = 1, n
     xf(i) = sin(0.3d0 * i)
     do k = 1, nmem
@@ -53,6 +59,13 @@ program oakg_fdriver
   Sa = 0
   call locAnalysis_gpu(zoneSize, select_gauss, xf, Hxf, yo, Sf, HSf, Rd, xa, Sa, info=info)
   print '(a,i0)', 'locAnalysis_gpu info = ', info
+  xa2 = 0
+  Sa2 = 0
+  info2 = info
+  if (info == 0) &
+    call locAnalysis_gpu(zoneSize, select_gauss, xf, Hxf, yo, Sf, HSf, Rd, xa2, Sa2, &
+         localise_obs=.false., info=info2)
+  print '(a,i0)', 'locAnalysis_gpu (localise_obs = .false.) info = ', info2
   ! the forecast report of Assim (assimilation.F90:3389), the last
   ! observation excluded
   allocate(excl(nobs))
@@ -62,9 +75,10 @@ program oakg_fdriver
     call report_gpu(6, 'forecast.', 55000.5d0, count(.not. excl), Rd, HSf, yo - Hxf, excl)
   if (len_trim(fname) > 0) then
     open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
-    write(u) int(info, 8), int(nzone, 8), int(zsize, 8), int(nmem, 8), int(nobs, 8)
-    write(u) xf, Hxf, yo, Sf, HSf, Rd, xa, Sa
+    write(u) int(info, 8), int(nzone, 8), int(zsize, 8), int(nmem, 8), int(nobs, 8), int(info2, 8)
+    write(u) xf, Hxf, yo, Sf, HSf, Rd, xa, Sa, xa2, Sa2
     close(u)
   end if
   if (info /= 0) call exit(info)
+  if (info2 /= 0) call exit(info2)
 end program oakg_fdriver

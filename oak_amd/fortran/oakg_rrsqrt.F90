@@ -3,9 +3,12 @@
 ! locAnalysis_gpu keeps locAnalysis' argument list; R is passed as its
 ! diagonal (the DiagCovar component R%D, covariance.F90:70-79), since OAK's
 ! local scheme uses a diagonal R.  The caller's selectObservations callback is
-! evaluated per zone on the host (zone i = 1..size(zoneSize), as
-! locAnalysisIncrement does, rrsqrt.F90:358-418), which keeps any selection
-! exact; the relevant observations and their weights go to the GPU as CSR.
+! evaluated once per zone on the host with the 1-based state index of the
+! zone's first element, as locAnalysisIncrement calls it
+! (startIndex(zi), rrsqrt.F90:358-368), which keeps any selection exact; the
+! relevant observations and their weights go to the GPU as CSR.  With
+! localise_obs = .false., a zone with at least one relevant observation uses
+! all observations with their weights (rrsqrt.F90:373-385).
 ! Like locAnalysis, it may be called by every thread of an OpenMP parallel
 ! region: the work runs in the master thread and ends with a barrier.
 module oakg_rrsqrt
@@ -55,7 +58,8 @@ contains
     type(oakg_obs) :: obs
     type(c_ptr) :: c, pampl
     integer(c_int) :: rc
-    integer :: nz, m, n, iz, k, p, nn
+    integer :: nz, m, n, iz, k, l
+    integer(c_int64_t) :: p, nn
     integer(c_int32_t) :: loc
 
 !$omp master
@@ -71,24 +75,31 @@ contains
     do iz = 1, nz
       zone_ptr(iz + 1) = zone_ptr(iz) + zoneSize(iz)
     end do
-    ! selection: count, then fill (0-based observation indices)
+    ! selection: one callback per zone with the zone's first state index,
+    ! its relevant observations appended to the CSR lists (0-based
+    ! observation indices; the lists grow by doubling)
+    nn = max(16, m)
+    allocate(sel_idx(nn), sel_w(nn))
     sel_ptr(1) = 0
-    do iz = 1, nz
-      call selectObservations(iz, wz, rel)
-      sel_ptr(iz + 1) = sel_ptr(iz) + count(rel)
-    end do
-    nn = int(sel_ptr(nz + 1))
-    allocate(sel_idx(max(nn, 1)), sel_w(max(nn, 1)))
     p = 0
     do iz = 1, nz
-      call selectObservations(iz, wz, rel)
-      do k = 1, m
-        if (rel(k)) then
+      call selectObservations(int(zone_ptr(iz)) + 1, wz, rel)
+      k = count(rel)
+      ! localise_obs = .false.: all observations (with their weights) as soon
+      ! as one is relevant (rrsqrt.F90:373-385)
+      if (loc == 0 .and. k > 0) then
+        rel = .true.
+        k = m
+      end if
+      if (p + k > nn) call grow(max(2 * nn, p + k))
+      do l = 1, m
+        if (rel(l)) then
           p = p + 1
-          sel_idx(p) = int(k - 1, c_int32_t)
-          sel_w(p) = wz(k)
+          sel_idx(p) = int(l - 1, c_int32_t)
+          sel_w(p) = wz(l)
         end if
       end do
+      sel_ptr(iz + 1) = p
     end do
     obs%m = m
     obs%yo = c_loc(yo)
@@ -113,6 +124,18 @@ contains
     end if
 !$omp end master
 !$omp barrier
+  contains
+    subroutine grow(cap)
+      integer(c_int64_t), intent(in) :: cap
+      integer(c_int32_t), allocatable, target :: i2(:)
+      real(c_double), allocatable, target :: w2(:)
+      allocate(i2(cap), w2(cap))
+      i2(1:p) = sel_idx(1:p)
+      w2(1:p) = sel_w(1:p)
+      call move_alloc(i2, sel_idx)
+      call move_alloc(w2, sel_w)
+      nn = cap
+    end subroutine grow
   end subroutine locAnalysis_gpu
 
   ! report (assimilation.F90:3584-3628) with the statistics computed on the

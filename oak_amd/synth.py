@@ -182,6 +182,76 @@ def lognormal_density(median_count, sigma_log, cutoff, corr_cells=60.0):
     return draw
 
 
+def calibrated_swaths(median=150.0, sigma_log=0.97, cutoff=18.0, n_prof=60, nz_lev=30,
+                      field_len=50.0, gap_len=80.0, gap_frac=0.12, bins=1000, iters=80):
+    """Observation positions whose local counts (observations within the
+    cutoff of each column) follow a log-normal distribution of the given
+    median and log standard deviation, with gaps of no data.
+
+    SSH observations are drawn from a cell density exp(T(u)) on the columns,
+    u the rank (0..1) of a smooth random field of correlation length
+    field_len: dense swaths where the field is high, sparse data where it is
+    low.  Cells where a second smooth field (gap_len) is in its
+    lowest gap_frac hold no data (neither SSH nor profiles), so the columns
+    deep inside these gaps have no local observation.  n_prof profile
+    columns of nz_lev values are drawn first, outside the gaps.  T is
+    calibrated by quantile matching: the expected local count of each column
+    (the density summed over the cutoff disc, by FFT convolution, plus the
+    profile values within the cutoff) is compared with the log-normal
+    quantile of its rank among the columns with data, and T is corrected per
+    rank bin of u, until the expected counts follow the log-normal; a final
+    scale puts their median on target.  Returns a callable for
+    column_ocean_obs(ssh_density=...) giving (sx, sy, px, py)."""
+    def draw(g, nx, ny):
+        from scipy.signal import fftconvolve
+        from scipy.stats import norm
+        f = grf(g, (nx, ny), (field_len, field_len), 1)[0]
+        h = grf(g, (nx, ny), (gap_len, gap_len), 1)[0]
+        mask = (h > np.quantile(h, gap_frac)).astype(np.float64)
+        # profiles: columns outside the gaps
+        px, py = [], []
+        while len(px) < n_prof:
+            x, y = g.uniform(1, nx), g.uniform(1, ny)
+            if mask[int(x) - 1, int(y) - 1] > 0:
+                px.append(x); py.append(y)
+        px = np.array(px); py = np.array(py)
+        ii, jj = np.meshgrid(np.arange(1, nx + 1, dtype=float), np.arange(1, ny + 1, dtype=float),
+                             indexing="ij")
+        prof = np.zeros((nx, ny))
+        for x, y in zip(px, py):
+            prof += nz_lev * ((ii - x) ** 2 + (jj - y) ** 2 <= cutoff ** 2)
+        R = int(np.ceil(cutoff)) + 1
+        yy, xx = np.mgrid[-R:R + 1, -R:R + 1]
+        disc = (xx ** 2 + yy ** 2 <= cutoff ** 2).astype(np.float64)
+        u = (np.argsort(np.argsort(f.ravel())).reshape(f.shape) + 0.5) / f.size
+        b = np.minimum((u * bins).astype(np.int64), bins - 1)
+        logT = np.log(median / disc.sum()) + sigma_log * norm.ppf((np.arange(bins) + 0.5) / bins)
+
+        def expected(logT):
+            return fftconvolve(np.exp(logT[b]) * mask, disc, mode="same") + prof
+        for _ in range(iters):
+            ec = expected(logT)
+            ok = ec > 0.5
+            rk = np.argsort(np.argsort(ec[ok]))
+            tgt = np.log(median) + sigma_log * norm.ppf((rk + 0.5) / ok.sum())
+            lr = np.zeros_like(ec)
+            lr[ok] = tgt - np.log(ec[ok])
+            s = np.bincount(b[ok], weights=lr[ok], minlength=bins)
+            c = np.bincount(b[ok], minlength=bins)
+            logT += 0.7 * np.where(c > 0, s / np.maximum(c, 1), 0.0)
+        for _ in range(20):  # the median of the columns with data on target
+            ec = expected(logT)
+            logT += np.log(median / np.median(ec[ec > 0.5]))
+        cnt = g.poisson(np.exp(logT[b]) * mask)
+        ci, cj = np.nonzero(cnt)
+        reps = cnt[ci, cj]
+        x = np.repeat(ci + 1.0, reps) + g.uniform(0, 1, reps.sum())
+        y = np.repeat(cj + 1.0, reps) + g.uniform(0, 1, reps.sum())
+        ok = (x < nx) & (y < ny)
+        return x[ok], y[ok], px, py
+    return draw
+
+
 def dense_H(P):
     """Dense m x n observation operator (small problems only)."""
     H = np.zeros((P.m, P.n))
@@ -280,7 +350,8 @@ def column_ocean_obs(g, nx, ny, nz_lev, truth, n_ssh, n_prof_cols, L, cutoff, si
         sx = np.concatenate(xs)[:n_ssh]
         sy = np.concatenate(ys)[:n_ssh]
     else:
-        sx, sy = ssh_density(g, nx, ny)
+        sxy = ssh_density(g, nx, ny)
+        sx, sy = sxy[0], sxy[1]
     nss = len(sx)
     i0, j0, w = bilinear(sx, sy, nx, ny)
     Hi = [np.repeat(np.arange(nss), 4)]
@@ -290,8 +361,13 @@ def column_ocean_obs(g, nx, ny, nz_lev, truth, n_ssh, n_prof_cols, L, cutoff, si
         cols.append(zc * s + ssh_base)
     Hj = [np.stack(cols, 1).ravel()]
     Hs = [w.ravel()]
-    px = g.uniform(1, nx, n_prof_cols)
-    py = g.uniform(1, ny, n_prof_cols)
+    if ssh_density is not None and len(sxy) == 4:
+        # the density drew the profile columns as well
+        px, py = sxy[2], sxy[3]
+        n_prof_cols = len(px)
+    else:
+        px = g.uniform(1, nx, n_prof_cols)
+        py = g.uniform(1, ny, n_prof_cols)
     pi0, pj0, pw = bilinear(px, py, nx, ny)
     npro = n_prof_cols * nz_lev
     Hi.append(np.repeat(nss + np.arange(npro), 4))
@@ -328,14 +404,15 @@ def config1_gpu(device="cuda:0"):
 
 
 def config2_gpu(N=128, device="cuda:0", dtype=None):
-    """config 1's grid with N = 128; SSH observations clustered by a smooth
-    log-normal density plus 60 T profiles of 30 levels.  The log-normal
-    parameters are set for the local counts within the 18-cell cutoff (the
-    disc average of the density is less variable than the density itself):
-    median ~145, P99.9 ~2800, range 0..~3000."""
+    """config 1's grid with N = 128; SSH observations in dense swaths and
+    sparse regions, 60 T profiles of 30 levels, and gaps without data, drawn
+    so that the local counts within the 18-cell cutoff are log-normal with
+    median 150 and log standard deviation 0.97 (calibrated_swaths): about 5 %
+    of the columns have no observation and the largest counts reach ~3000."""
     return column_ocean_gpu(360, 180, 30, N, 2, n_prof_cols=60,
-                            ssh_density=lognormal_density(145.0, 1.2, 18.0), dtype=dtype,
-                            device=device)
+                            ssh_density=calibrated_swaths(150.0, 0.97, 18.0, n_prof=60,
+                                                          nz_lev=30),
+                            dtype=dtype, device=device)
 
 
 def config3(nx=256, N=32, m=4000, seed_cfg=3):

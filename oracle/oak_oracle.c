@@ -573,6 +573,8 @@ void oko_obsoper(int64_t m, int64_t nnz, const int32_t* Hi, const int32_t* Hj, c
 /*   inflation, maxCorrection clamp, ensemble rebuild and re-centring          */
 /*   (assimilation.F90:3299-3357, 3558-3563).                                 */
 /* E, Ea: n x N (ld n), in zone-permuted order; H columns already permuted.    */
+/* maxCorrVec: OAK's per-element maxCorrection (assimilation.F90:462-476,      */
+/* permuted like the state), or NULL to use the scalar maxCorrection for all.  */
 /* Zones outside [z0,z1) are left without analysis (testing only).           */
 /* ------------------------------------------------------------------------- */
 int oko_ensemble_analysis(int nz, const int64_t* zone_ptr, int z0, int z1, int N, int64_t m,
@@ -582,8 +584,8 @@ int oko_ensemble_analysis(int nz, const int64_t* zone_ptr, int z0, int z1, int N
                           int64_t nnz, const int32_t* Hi, const int32_t* Hj, const double* Hs,
                           const double* Hshift, const double* yo, const double* sig2,
                           const double* dex, const double* E, double inflation,
-                          double maxCorrection, double* Ea, double* xa_out, int nthreads,
-                          int64_t* counts) {
+                          double maxCorrection, const double* maxCorrVec, double* Ea,
+                          double* xa_out, int nthreads, int64_t* counts) {
   int64_t n = zone_ptr[nz];
   double scaling = sqrt((double)N - 1.);
   double* HE = (double*)malloc(sizeof(double) * (size_t)m * N);
@@ -619,8 +621,9 @@ int oko_ensemble_analysis(int nz, const int64_t* zone_ptr, int z0, int z1, int N
   if (inflation != 1)
     for (size_t i = 0; i < (size_t)n * N; ++i) Ea[i] = inflation * Ea[i];
   for (int64_t i = 0; i < n; ++i) {
-    if (xa[i] - maxCorrection > xf[i]) xa[i] = xf[i] + maxCorrection;
-    if (xa[i] < xf[i] - maxCorrection) xa[i] = xf[i] - maxCorrection;
+    const double mc = maxCorrVec ? maxCorrVec[i] : maxCorrection;
+    if (xa[i] - mc > xf[i]) xa[i] = xf[i] + mc;
+    if (xa[i] < xf[i] - mc) xa[i] = xf[i] - mc;
   }
   for (int k = 0; k < N; ++k)
     for (int64_t i = 0; i < n; ++i) Ea[i + (size_t)k * n] = xa[i] + Ea[i + (size_t)k * n] * scaling;

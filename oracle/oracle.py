@@ -245,7 +245,8 @@ def ensemble_analysis(geo, H, Hshift, yo, sig2, E, dex=None, inflation=1.0,
                       max_correction=np.inf, zones=None, nthreads=0):
     """Assim's ensemble path with the local scheme (assimilation.F90:3094-3135,
     3299-3357, 3558-3563).  H = (Hi, Hj, Hs) COO, 0-based, columns in
-    zone-permuted order.  Returns (Ea, xa, counts)."""
+    zone-permuted order.  max_correction: a scalar, or OAK's per-element
+    maxCorrection array (n, permuted).  Returns (Ea, xa, counts)."""
     zone_ptr = np.ascontiguousarray(geo["zone_ptr"], dtype=np.int64)
     nz = len(zone_ptr) - 1
     z0, z1 = zones if zones is not None else (0, nz)
@@ -261,12 +262,20 @@ def ensemble_analysis(geo, H, Hshift, yo, sig2, E, dex=None, inflation=1.0,
     xa = np.zeros(n)
     counts = np.zeros(nz, dtype=np.int64)
     dexa = None if dex is None else _f(dex)
+    if np.ndim(max_correction) == 0:
+        mcv = None
+    else:
+        mcv = _f(max_correction)
+        max_correction = np.inf
+        if len(mcv) != n:
+            raise OracleError("max_correction: one value per state element")
     rc = lib().oko_ensemble_analysis(
         C.c_int(nz), _ptr(zone_ptr), C.c_int(z0), C.c_int(z1), C.c_int(N), C.c_int64(m),
         C.c_int(geo["metric"]), C.c_int(geo["loctype"]), _ptr(zpos), _ptr(L), _ptr(cut),
         _ptr(ox), _ptr(oy), _ptr(oz), _ptr(ot), C.c_int64(len(Hs)), _ptr(Hi), _ptr(Hj), _ptr(Hs),
         _ptr(_f(Hshift)), _ptr(_f(yo)), _ptr(_f(sig2)), _ptr(dexa), _ptr(E),
-        C.c_double(inflation), C.c_double(max_correction), _ptr(Ea), _ptr(xa), C.c_int(nthreads),
+        C.c_double(inflation), C.c_double(max_correction), _ptr(mcv), _ptr(Ea), _ptr(xa),
+        C.c_int(nthreads),
         _ptr(counts))
     if rc:
         raise OracleError("ensemble_analysis rc=%d" % rc)

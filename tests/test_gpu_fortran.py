@@ -18,11 +18,20 @@ def test_fortran_locanalysis_vs_oracle(tmp_path):
     r = subprocess.run([DRIVER, str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     d = read(out)
-    assert d["info"] == 0
-    xa, Sa = oracle_analysis(d)
-    dx = np.abs(d["xa"] - xa).max() / np.abs(xa - d["xf"]).max()
-    ds = np.abs(d["Sa"] - Sa).max() / np.abs(Sa).max()
-    assert dx < 1e-10 and ds < 1e-10, (dx, ds)
+    assert d["info"] == 0 and d["info2"] == 0
+    for loc, kx, ks in ((True, "xa", "Sa"), (False, "xa2", "Sa2")):
+        xa, Sa = oracle_analysis(d, localise_obs=loc)
+        dx = np.abs(d[kx] - xa).max() / np.abs(xa - d["xf"]).max()
+        ds = np.abs(d[ks] - Sa).max() / np.abs(Sa).max()
+        assert dx < 1e-10 and ds < 1e-10, (loc, dx, ds)
+    # the last zone has no observation within the cutoff: untouched
+    last = slice(d["n"] - d["zsize"], d["n"])
+    assert np.array_equal(d["xa"][last], d["xf"][last])
+    assert np.array_equal(d["Sa2"][last], d["Sf"][last])
+    # the callback called with the zone number instead of the first state
+    # index (the mistake this pins) gives a different analysis
+    xz, _ = oracle_analysis(d, zone_number=True)
+    assert np.abs(d["xa"] - xz).max() > 1e-3 * np.abs(xz - d["xf"]).max()
 
 
 def test_fortran_report_vs_oracle(tmp_path):

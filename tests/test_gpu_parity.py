@@ -368,7 +368,7 @@ def test_host_leading_dimensions(oak, cfg0):
     ctx = default_context()
     rc = _lib.load().oakg_ensemble_analysis(ctx.handle, C.byref(z), C.byref(o), C.byref(h),
                                             hs.ctypes.data, N, Ef.ctypes.data, ld,
-                                            Eo.ctypes.data, ld, 1.0, float("inf"), 0)
+                                            Eo.ctypes.data, ld, 1.0, float("inf"), None, 0)
     ctx.check(rc)
     np.testing.assert_array_equal(Eo[:n], Ea)
     assert np.all(Eo[n:] == -7.0)  # rows beyond n untouched

@@ -137,6 +137,53 @@ def test_perp_space_and_rotation(sign):
     assert rel_err(O.rotate_vector(w, w), np.eye(n)) < 1e-14
 
 
+@pytest.mark.parametrize("case", ["random+", "random-", "+e_N", "-e_N", "+0 last"])
+def test_perp_space_is_a_householder_reflector(case):
+    """perpSpace(u) (matoper.F90:509-535) is the first N-1 columns of the
+    reflector I - 2 z z^T / z^T z with z = u + sign(u_N) e_N, Fortran's
+    sign(1., u_N) = +1 for u_N >= +0 (written here from the reflector, not
+    from the closed form of the oracle).  Any other orthonormal complement
+    (a QR-based basis, the opposite sign convention) fails this, and it is
+    what fixes the individual analysed members."""
+    rng = np.random.default_rng(17)
+    n = 11
+    if case.startswith("random"):
+        u = rng.standard_normal(n)
+        u[-1] = abs(u[-1]) * (1 if case.endswith("+") else -1)
+        u /= np.linalg.norm(u)
+    elif case == "+0 last":
+        u = rng.standard_normal(n); u[-1] = 0.0; u /= np.linalg.norm(u)
+    else:
+        u = np.zeros(n); u[-1] = 1.0 if case[0] == "+" else -1.0
+    s = 1.0 if u[-1] >= 0 else -1.0
+    z = u.copy(); z[-1] += s
+    Hh = np.eye(n) - 2.0 * np.outer(z, z) / (z @ z)
+    P = O.perp_space(u)
+    assert np.max(np.abs(P - Hh[:, : n - 1])) < 1e-15
+    if case.endswith("e_N"):
+        assert np.max(np.abs(P - np.eye(n)[:, : n - 1])) == 0.0
+
+
+def test_rotate_vector_members_from_the_reflectors():
+    """RotateVector(w, v) = v w^T + P(v) P(w)^T (rrsqrt.F90:737-756) with
+    P(.) the reflector columns above, applied to an ensemble: the analysed
+    members depend on this particular basis, so Omega is pinned member by
+    member (not only by Omega w = v and orthogonality)."""
+    rng = np.random.default_rng(19)
+    n = 7
+
+    def refl(u):
+        s = 1.0 if u[-1] >= 0 else -1.0
+        z = u.copy(); z[-1] += s
+        return (np.eye(n) - 2.0 * np.outer(z, z) / (z @ z))[:, : n - 1]
+    for _ in range(5):
+        w = rng.standard_normal(n); w /= np.linalg.norm(w)
+        v = rng.standard_normal(n); v /= np.linalg.norm(v)
+        Om = O.rotate_vector(w, v)
+        ref = np.outer(v, w) + refl(v) @ refl(w).T
+        assert np.max(np.abs(Om - ref)) < 1e-14
+
+
 # --- rrsqrt.F90:100-190 against the textbook KF (test/test_rrsqrt.F90:27-77) ---
 def test_analysis_global_kf(backend):
     d = rrsqrt_inputs()
@@ -298,6 +345,72 @@ def test_ensemble_analysis_geometry_against_dense_kf(backend):
         xa_z, Pa_z = local_kf_rows(xf, Sf, Hd, yo, sig2, [z], idx, w)
         assert abs(xa[z] - xa_z[0]) < 1e-10 * max(1, abs(xa_z[0]))
         assert abs(Sa[z] @ Sa[z] - Pa_z[0, 0]) < 1e-10 * max(1, Pa_z[0, 0])
+
+
+@pytest.mark.parametrize("per_element", [False, True])
+def test_post_processing_inflation_and_clamp_against_dense_kf(backend, per_element):
+    """Assim's post-processing (assimilation.F90:3299-3320, 3347-3357,
+    462-476) against the per-zone dense Kalman filter: with inflation 1.3 and
+    a maxCorrection that binds, mean(Ea) = clip(xa_KF, xf +- maxCorrection)
+    and var(Ea, ddof=1) = inflation^2 Pa_KF (the clamp moves the mean only;
+    inflation scales the spread only); zones without observations keep
+    inflation^2 Pf.  Two variables per zone (A observed, B correlated with A)
+    with their own clamps in the per-element case.  Inflating the mean,
+    clamping the members instead of xa, or inflating before the transform's
+    re-centring fail this."""
+    rng = np.random.default_rng(23)
+    nx, ny, N, m = 9, 7, 12, 16
+    npt = nx * ny
+    n = 2 * npt
+    A0 = rng.standard_normal((npt, N))
+    B0 = 0.7 * A0 + 0.5 * rng.standard_normal((npt, N))
+    E = np.empty((n, N))
+    E[0::2] = A0 + 0.2
+    E[1::2] = B0 - 0.1
+    gx, gy = np.meshgrid(np.arange(nx, dtype=float), np.arange(ny, dtype=float), indexing="ij")
+    gx = gx.ravel(order="F"); gy = gy.ravel(order="F")
+    # observations in the left two thirds only: the right columns have no
+    # observation within the cutoff
+    ox = rng.uniform(0, 4.5, m); oy = rng.uniform(0, ny - 1, m)
+    Hi, Hj, Hs = [], [], []
+    for l in range(m):
+        i0 = min(int(ox[l]), nx - 2); j0 = min(int(oy[l]), ny - 2)
+        a = ox[l] - i0; b = oy[l] - j0
+        for di, dj, c in ((0, 0, (1 - a) * (1 - b)), (1, 0, a * (1 - b)), (0, 1, (1 - a) * b),
+                          (1, 1, a * b)):
+            Hi.append(l); Hj.append(2 * ((i0 + di) + nx * (j0 + dj))); Hs.append(c)
+    Hi = np.array(Hi); Hj = np.array(Hj); Hs = np.array(Hs)
+    Hd = np.zeros((m, n)); np.add.at(Hd, (Hi, Hj), Hs)
+    yo = rng.standard_normal(m) * 2.0 + 1.0
+    sig2 = np.full(m, 0.04); Hshift = np.zeros(m)
+    infl = 1.3
+    mc = np.tile([0.05, 0.2], npt) if per_element else 0.08
+    geo = dict(zone_ptr=2 * np.arange(npt + 1), zpos=np.stack([gx, gy, 0 * gx, 0 * gx], 1),
+               L=np.full(npt, 1.5), cutoff=np.full(npt, 3.0), metric=0, loctype=1, ox=ox, oy=oy)
+    Ea, xa, counts = O.ensemble_analysis(geo, (Hi, Hj, Hs), Hshift, yo, sig2, E,
+                                         inflation=infl, max_correction=mc)
+    xf = E.mean(axis=1); Sf = (E - xf[:, None]) / np.sqrt(N - 1)
+    mcv = np.broadcast_to(mc, (n,))
+    nclamp = nempty = 0
+    for z in range(npt):
+        rows = [2 * z, 2 * z + 1]
+        d = np.hypot(ox - gx[z], oy - gy[z])
+        idx = np.nonzero(d <= 3.0)[0]
+        assert counts[z] == len(idx)
+        if len(idx) == 0:
+            nempty += 1
+            xk, Pk = xf[rows], (Sf[rows] @ Sf[rows].T)
+        else:
+            w = np.exp(-(d[idx] / 1.5) ** 2)
+            xk, Pk = local_kf_rows(xf, Sf, Hd, yo, sig2, rows, idx, w)
+        xc = np.clip(xk, xf[rows] - mcv[rows], xf[rows] + mcv[rows])
+        nclamp += int(np.sum(xc != xk))
+        assert np.max(np.abs(Ea[rows].mean(axis=1) - xc)) < 1e-10 * max(1, np.abs(xc).max())
+        var = Ea[rows].var(axis=1, ddof=1)
+        assert np.max(np.abs(var - infl ** 2 * np.diag(Pk))) < 1e-10 * max(1, np.diag(Pk).max())
+        cov = np.cov(Ea[rows])
+        assert abs(cov[0, 1] - infl ** 2 * Pk[0, 1]) < 1e-10 * max(1, np.abs(Pk).max())
+    assert nclamp > 10 and nempty > 5, (nclamp, nempty)
 
 
 # --- report, assimilation.F90:3584-3628 ---

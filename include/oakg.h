@@ -129,7 +129,8 @@ int oakg_destroy(oakg_ctx* ctx);
 const char* oakg_last_error(const oakg_ctx* ctx);
 /* Launch on this HIP stream (hipStream_t) instead of the default stream.
    With 4096 zones or more (and phase timing off) the zone chunks also run on
-   two internal non-blocking streams of the context (default priority),
+   internal non-blocking streams of the context (default priority; two with
+   the default OAKG_TUNE_SLOTS),
    ordered after the work already on `stream` and joined back into it by
    events before the call returns; the call synchronises `stream`. */
 int oakg_set_stream(oakg_ctx* ctx, void* stream);
@@ -145,6 +146,25 @@ int oakg_set_timing(oakg_ctx* ctx, int on);
 #define OAKG_EIG_QL 1
 int oakg_set_eigensolver(oakg_ctx* ctx, int method);
 int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
+/* Tuning knobs of the zone loop (results do not depend on them; each zone is
+ * computed alone, so every setting gives the same bits):
+ *   OAKG_TUNE_SLOTS       zone chunks in flight on as many streams (1..4,
+ *                         default 3; from 4096 zones, phase timing off)
+ *   OAKG_TUNE_QL_ZONES    zones per wave of the QL kernel (8, 16, 32; 16)
+ *   OAKG_TUNE_GRAPH       1: the zone loop (Gram, eigensolve, transform,
+ *                         update of every chunk, on all its streams) is
+ *                         captured in a HIP graph on first use and replayed
+ *                         while its kernel arguments and buffers stay the
+ *                         same (a caller that repeats an analysis on the same
+ *                         device buffers); 0: launched eagerly (default)
+ * oakg_get_tuning also reads OAKG_TUNE_GRAPH_REPLAYS / _CAPTURES (counts). */
+#define OAKG_TUNE_SLOTS 1
+#define OAKG_TUNE_QL_ZONES 2
+#define OAKG_TUNE_GRAPH 3
+#define OAKG_TUNE_GRAPH_REPLAYS 4
+#define OAKG_TUNE_GRAPH_CAPTURES 5
+int oakg_set_tuning(oakg_ctx* ctx, int key, int value);
+int oakg_get_tuning(oakg_ctx* ctx, int key, int64_t* value);
 /* Per-zone local observation counts p_z of the last call (host array). */
 int oakg_get_counts(oakg_ctx* ctx, int64_t* counts, int64_t nzones);
 
@@ -167,6 +187,7 @@ int oakg_local_analysis_csr(oakg_ctx* ctx, int64_t nzones, const int64_t* zone_p
                             const double* Sf, int64_t ldSf, const double* HSf, int64_t ldHSf,
                             double* xa, double* Sa, int64_t ldSa, double* amplitudes,
                             int32_t localise_obs, uint32_t flags);
+
 
 /* The selection alone (assimilation.F90:3683-3771) for every zone: writes
  * sel_ptr[nzones+1] and, if sel_idx/sel_w are not NULL, up to cap entries of
@@ -253,6 +274,58 @@ int oakg_local_nll(oakg_ctx* ctx, const oakg_zones* zones, const oakg_obs* obs, 
                    const double* Hxf, const double* HSf, int64_t ldHSf, double inflation,
                    double* nll, double* grad_log_sigma, double* grad_corr_len,
                    double* grad_inflation, double* nll_zone, uint32_t flags);
+
+/* A model variable's grid for the observation operator (ModelGrid(v) of
+ * assimilation.F90:335-361, with explicit coordinates as OAK's grid type
+ * keeps them, and the memory layout of the variable in the state vector). */
+typedef struct {
+  int32_t ndim;               /* 1..4 (ModML%ndim(v))                                */
+  int64_t shape[4];           /* nodes per dimension, at least 2 each (varshape)     */
+  const double* coord[4];     /* [prod(shape)] node coordinate along each dimension, */
+                              /*   first index fastest (the files of Model.gridX/Y/Z/T) */
+  const uint8_t* masked;      /* [prod(shape)] 1 = land / invalid node, or NULL      */
+  const int64_t* state_index; /* [prod(shape)] 0-based column of H for each node: the */
+                              /*   state index after SeaIndex and the zone permutation */
+                              /*   (assimilation.F90:1781-1814, 2404-2409), -1 = not in */
+                              /*   the state; NULL = the node index itself              */
+} oakg_grid;
+
+/* genObservationOper (assimilation.F90:2471-2656) + packSparseMatrix
+ * (:4132-4208) + the zone permutation of H's columns (:2404-2409) on the
+ * GPU: observation l of model variable obs_var[l] (0-based; -1 or >= nvar:
+ * unknown) at (ox, oy, oz, ot)[l] (only the first ndim are read) is located
+ * in its grid by OAK's box search (ndgrid_inc.F90:921-990) and interpolated
+ * by cinterp's hyper-tetrahedral weights (ndgrid.F90:357-665, 1183-1257):
+ * row l of H holds the 2^ndim corners of its cell (cinterp's order), or
+ * nothing when it is outside the grid, next to a masked node or of an
+ * unknown variable.  Entries of nodes outside the state are dropped
+ * (packSparseMatrix).  Outputs: row_ptr[m+1] (CSR by observation), up to cap
+ * entries col/val, valid[m] (0: OAK excludes the observation, validobs of
+ * loadObservationOper :2382-2387) and *nnz (the entries H holds; with
+ * *nnz > cap nothing past cap is written and OAKG_EARG is returned).
+ * Cells whose simplices are degenerate (|det| <= 1e-8, e.g. a dimension of
+ * one node; OAK's SVD branch, ndgrid.F90:514-627) return OAKG_EUNSUP. */
+int oakg_obs_operator(oakg_ctx* ctx, int32_t nvar, const oakg_grid* grids, int64_t m,
+                      const int32_t* obs_var, const double* ox, const double* oy,
+                      const double* oz, const double* ot, int64_t* row_ptr, int32_t* col,
+                      double* val, int64_t cap, uint8_t* valid, int64_t* nnz, uint32_t flags);
+
+/* The zone-permuted state and the model's field files (loadVector /
+ * loadEnsemble / saveVector with ModML%permute, assimilation.F90:869-880,
+ * 901, 1268-1359): with idx[l] = invindex(zoneIndex(l)) (0-based; the full
+ * field index, land included, of permuted state row l),
+ *   oakg_gather_rows:  out[l + k ldo] = in[idx[l] + k ldi]
+ *   oakg_scatter_rows: out[idx[l] + k ldo] = in[l + k ldi]
+ * for l < n and members k < N; elem_size 4 (FP32) or 8 (FP64); full_len is
+ * the number of rows of the full field array (every idx must lie in
+ * [0, full_len), else OAKG_EARG); rows of the scatter's out that no idx
+ * names are left as they are (land values). */
+int oakg_gather_rows(oakg_ctx* ctx, int64_t n, int32_t N, const int64_t* idx, int64_t full_len,
+                     const void* in, int64_t ldi, void* out, int64_t ldo, int32_t elem_size,
+                     uint32_t flags);
+int oakg_scatter_rows(oakg_ctx* ctx, int64_t n, int32_t N, const int64_t* idx, int64_t full_len,
+                      const void* in, int64_t ldi, void* out, int64_t ldo, int32_t elem_size,
+                      uint32_t flags);
 
 /* Host helpers ------------------------------------------------------------ */
 /* Stable COO -> CSR by row (entries of a row keep their COO order). */

@@ -25,7 +25,11 @@ EXPORTS = [
     "oakg_local_analysis_csr", "oakg_select_observations", "oakg_ensemble_analysis",
     "oakg_ensemble_analysis_f32", "oakg_local_nll", "oakg_report", "oakg_ensemble_report",
     "oakg_ensemble_report_f32", "oakg_coo_to_csr", "oakg_partition",
+    "oakg_obs_operator", "oakg_gather_rows",
+    "oakg_scatter_rows", "oakg_set_tuning", "oakg_get_tuning",
 ]
+OAKG_TUNE_SLOTS, OAKG_TUNE_QL_ZONES, OAKG_TUNE_GRAPH = 1, 2, 3
+OAKG_TUNE_GRAPH_REPLAYS, OAKG_TUNE_GRAPH_CAPTURES = 4, 5
 
 
 class OakError(RuntimeError):
@@ -56,6 +60,11 @@ class Obs(C.Structure):
 class Csr(C.Structure):
     _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("row_ptr", _vp),
                 ("col", _vp), ("val", _vp)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("shape", C.c_int64 * 4), ("coord", _vp * 4),
+                ("masked", _vp), ("state_index", _vp)]
 
 
 class Stats(C.Structure):
@@ -100,6 +109,8 @@ def load():
     L.oakg_set_timing.argtypes = [_vp, C.c_int]
     L.oakg_set_eigensolver.argtypes = [_vp, C.c_int]
     L.oakg_get_stats.argtypes = [_vp, C.POINTER(Stats)]
+    L.oakg_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+    L.oakg_get_tuning.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64)]
     L.oakg_get_counts.argtypes = [_vp, _vp, C.c_int64]
     L.oakg_local_analysis.argtypes = [
         _vp, C.POINTER(Zones), C.POINTER(Obs), C.c_int32, _vp, _vp, _vp, C.c_int64, _vp,
@@ -124,6 +135,12 @@ def load():
         C.c_uint32]
     L.oakg_ensemble_report_f32.argtypes = L.oakg_ensemble_report.argtypes
     L.oakg_coo_to_csr.argtypes = [C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.oakg_gather_rows.argtypes = [_vp, C.c_int64, C.c_int32, _vp, C.c_int64, _vp, C.c_int64, _vp,
+                                   C.c_int64, C.c_int32, C.c_uint32]
+    L.oakg_scatter_rows.argtypes = L.oakg_gather_rows.argtypes
+    L.oakg_obs_operator.argtypes = [
+        _vp, C.c_int32, C.POINTER(Grid), C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+        C.c_int64, _vp, C.POINTER(C.c_int64), C.c_uint32]
     L.oakg_partition.restype = C.c_int64
     L.oakg_partition.argtypes = [C.c_int64, _vp, _vp, _vp, _vp]
     _lib = L
@@ -173,6 +190,26 @@ class Context:
     def set_eigensolver(self, method):
         """OAKG_EIG_QL (default) or OAKG_EIG_JACOBI."""
         self.check(load().oakg_set_eigensolver(self._h, int(method)))
+
+    def set_tuning(self, slots=None, ql_zones=None, graph=None):
+        """Zone chunks in flight, zones per QL wave, HIP-graph replay of the
+        zone loop (oakg_set_tuning)."""
+        L = load()
+        for key, v in ((OAKG_TUNE_SLOTS, slots), (OAKG_TUNE_QL_ZONES, ql_zones),
+                       (OAKG_TUNE_GRAPH, graph)):
+            if v is not None:
+                self.check(L.oakg_set_tuning(self._h, key, int(v)))
+
+    def tuning(self):
+        L = load()
+        out = {}
+        for name, key in (("slots", OAKG_TUNE_SLOTS), ("ql_zones", OAKG_TUNE_QL_ZONES),
+                          ("graph", OAKG_TUNE_GRAPH), ("graph_replays", OAKG_TUNE_GRAPH_REPLAYS),
+                          ("graph_captures", OAKG_TUNE_GRAPH_CAPTURES)):
+            v = C.c_int64(0)
+            self.check(L.oakg_get_tuning(self._h, key, C.byref(v)))
+            out[name] = v.value
+        return out
 
     def set_stream(self, stream_handle):
         self.check(load().oakg_set_stream(self._h, stream_handle))

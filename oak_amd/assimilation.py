@@ -179,6 +179,63 @@ def coo_to_csr(m, rows, cols, vals):
     return rp, co[:nnz], va[:nnz]
 
 
+def obs_operator(grids, obs_var, positions, ctx=None):
+    """genObservationOper + packSparseMatrix (assimilation.F90:2471-2656,
+    4132-4208) on the GPU: the observation operator of observations at
+    ``positions`` (up to 4 arrays [m]: x, y, z, t) of model variables
+    ``obs_var`` [m] (-1: unknown) from the model grids' coordinates.
+
+    grids: list of dicts, one per model variable, with 'coords' (ndim arrays
+    of the grid shape, Fortran order: first index fastest, as OAK reads
+    Model.gridX/Y/Z/T), optional 'mask' (True = land) and 'state' (node ->
+    0-based column of H in the permuted state, -1 = not in the state; by
+    default the node index).  Host (numpy) arrays.
+
+    Returns (row_ptr, col, val, valid): H in CSR by observation (the
+    hyper-tetrahedral weights of the 2^ndim corners of each observation's
+    cell, cinterp's order) and validobs (False: OAK excludes the
+    observation)."""
+    ctx = ctx or default_context()
+    L = _lib.load()
+    keep = []
+    gs = (_lib.Grid * max(len(grids), 1))()
+    twon = 1
+    for v, g in enumerate(grids):
+        coords = g["coords"]
+        n = len(coords)
+        twon = max(twon, 2 ** n)
+        gs[v].ndim = n
+        for d in range(4):
+            gs[v].shape[d] = coords[0].shape[d] if d < n else 1
+        for d in range(n):
+            a = np.ascontiguousarray(np.asarray(coords[d], np.float64).ravel(order="F"))
+            keep.append(a)
+            gs[v].coord[d] = a.ctypes.data
+        if g.get("mask") is not None:
+            a = np.ascontiguousarray(np.asarray(g["mask"], np.uint8).ravel(order="F"))
+            keep.append(a)
+            gs[v].masked = a.ctypes.data
+        if g.get("state") is not None:
+            a = np.ascontiguousarray(np.asarray(g["state"], np.int64).ravel(order="F"))
+            keep.append(a)
+            gs[v].state_index = a.ctypes.data
+    obs_var = np.ascontiguousarray(obs_var, np.int32)
+    m = len(obs_var)
+    pos = [np.ascontiguousarray(p, np.float64) for p in positions]
+    pp = [p.ctypes.data for p in pos] + [None] * (4 - len(pos))
+    row_ptr = np.zeros(m + 1, np.int64)
+    valid = np.zeros(max(m, 1), np.uint8)
+    cap = max(1, m * twon)
+    col = np.zeros(cap, np.int32)
+    val = np.zeros(cap)
+    nnz = C.c_int64(0)
+    ctx.check(L.oakg_obs_operator(ctx.handle, len(grids), gs, m, obs_var.ctypes.data, pp[0],
+                                  pp[1], pp[2], pp[3], row_ptr.ctypes.data, col.ctypes.data,
+                                  val.ctypes.data, cap, valid.ctypes.data, C.byref(nnz), 0))
+    k = nnz.value
+    return row_ptr, col[:k], val[:k], valid[:m].astype(bool)
+
+
 def partition(part):
     """initPartition (assimilation.F90:578-641) after the relabelling of
     :408-412.  Returns (zone_size, zone_index, inv_zone_index), 0-based."""

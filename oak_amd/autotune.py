@@ -1,0 +1,76 @@
+"""Autotuner of the zone loop's launch shapes.
+
+The knobs (oakg_set_tuning) change how the zones are scheduled, never the
+results: the number of zone chunks in flight on as many streams, the number
+of zones per wave of the QL kernel, and whether the zone loop is replayed
+from a HIP graph.  ``tune`` times every candidate on the caller's own
+workload (device-resident inputs, the same call repeated), keeps the
+fastest, and returns a table entry; ``apply`` sets a context from a table
+(a JSON file keyed by ensemble size and precision, with the workload it was
+tuned on)."""
+import itertools
+import json
+import os
+import time
+
+CANDIDATES = {"slots": (1, 2, 3, 4), "ql_zones": (8, 16, 32), "graph": (0, 1)}
+DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0}
+
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+
+def key(N, precision):
+    return "N%d_%s" % (int(N), "mixed" if int(precision) == 1 else "fp64")
+
+
+def tune(run, ctx, repeats=2, space=None, log=None):
+    """run(): one call of the analysis with ctx (synchronised by the caller's
+    function).  Every point of the space is run once untimed, then timed
+    over ``repeats`` calls; returns (best settings, {settings: ms})."""
+    import torch
+    space = space or CANDIDATES
+    names = list(space)
+    times = {}
+    for vals in itertools.product(*(space[n] for n in names)):
+        s = dict(zip(names, vals))
+        ctx.set_tuning(**s)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / repeats
+        times[json.dumps(s, sort_keys=True)] = ms
+        if log:
+            log("# autotune %s: %.3f ms" % (s, ms))
+    best = json.loads(min(times, key=times.get))
+    ctx.set_tuning(**best)
+    return best, times
+
+
+def load_table(path=TABLE):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def apply(ctx, N, precision, path=TABLE):
+    """Sets ctx from the table entry of (N, precision); the defaults when
+    the table has none.  Returns the settings applied."""
+    ent = load_table(path).get(key(N, precision))
+    s = dict(DEFAULT)
+    if ent:
+        s.update({k: ent["settings"][k] for k in DEFAULT if k in ent["settings"]})
+    ctx.set_tuning(**s)
+    return s
+
+
+def save_entry(N, precision, settings, times, workload, path=TABLE):
+    tab = load_table(path)
+    tab[key(N, precision)] = {"settings": settings, "workload": workload,
+                              "ms": {k: round(v, 3) for k, v in sorted(times.items(),
+                                                                       key=lambda kv: kv[1])}}
+    with open(path, "w") as f:
+        json.dump(tab, f, indent=1, sort_keys=True)

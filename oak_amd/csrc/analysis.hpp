@@ -85,6 +85,7 @@ struct ChunkArgs {
   int32_t* sweeps;
   // eigensolver: 0 = one-sided Jacobi, 1 = Householder tridiagonalisation + implicit QL
   int eig_method;
+  int ql_zw;                // zones per wave of the QL kernel (8, 16 or 32)
   double *dvec, *evec;      // tridiagonal / eigenvalues, NP x nzc
   void* rlog;               // QL rotation log, rlog_cap records of 16 B per zone
   int64_t rlog_cap;

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -15,29 +16,11 @@
 #include "analysis.hpp"
 #include "common.hpp"
 #include "internal.hpp"
+#include "interp.hpp"
 
 #ifndef OAKG_PIPE_SLOTS
-#define OAKG_PIPE_SLOTS 3  // zone chunks in flight on as many streams
+#define OAKG_PIPE_SLOTS 4  // at most this many zone chunks in flight on as many streams
 #endif
-
-namespace oakg {
-// Bump allocator over a fixed device buffer (the no-allocation entry point's
-// scratch array): buffers of one call, 256-byte aligned, released together.
-struct Arena {
-  char* base = nullptr;
-  size_t cap = 0, used = 0;
-  void* get(size_t bytes) {
-    const size_t at = (used + 255) & ~(size_t)255;
-    if (at > cap || bytes > cap - at) return nullptr;
-    used = at + bytes;
-    return base + at;
-  }
-  size_t left() const {
-    const size_t at = (used + 255) & ~(size_t)255;
-    return at < cap ? cap - at : 0;
-  }
-};
-}  // namespace oakg
 
 struct oakg_ctx {
   int device = 0;
@@ -62,11 +45,19 @@ struct oakg_ctx {
   // internal streams of the chunk pipeline (default priority) and their events
   hipStream_t aux[OAKG_PIPE_SLOTS - 1] = {};
   hipEvent_t ev_fork = nullptr, ev_join[OAKG_PIPE_SLOTS - 1] = {};
+  // tuning knobs (oakg_set_tuning): zone chunks in flight, zones per QL wave
+  int tune_slots = 3;
+  int tune_ql_zw = 16;
+  // the zone loop captured in a HIP graph and replayed while its launches
+  // (kernel arguments, buffers, streams) stay the same
+  bool graph = false;
+  hipStream_t cap_stream = nullptr;
+  hipEvent_t ev_cap_in = nullptr, ev_cap_out = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;
+  int64_t graph_replays = 0, graph_captures = 0;
   // events around each call's device work (its ms_total), created once
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
-  // set only during a call of oakg_local_analysis_csr_noalloc: every buffer
-  // of the call comes from this arena (the device's scratch array)
-  oakg::Arena* arena = nullptr;
 };
 
 namespace oakg {
@@ -135,6 +126,14 @@ void fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
 static void fill_f64(double* p, int64_t n, double v, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
@@ -162,10 +161,8 @@ void arg_fail(oakg_ctx* c, const std::string& msg) {
   throw Fail{OAKG_EARG};
 }
 
-// A device buffer of the call: from the context's grow-only workspace, or,
-// during a no-allocation call, from the scratch arena (nullptr when full).
+// A device buffer of the call, from the context's grow-only workspace.
 void* wsget(oakg_ctx* c, Workspace::Slot s, size_t bytes) {
-  if (c->arena) return c->arena->get(bytes == 0 ? 8 : bytes);
   return c->ws.get(s, bytes);
 }
 
@@ -293,17 +290,12 @@ void validate_obs(oakg_ctx* c, const oakg_obs* o, bool need_geo, int loctype) {
 void begin_call(oakg_ctx* c) {
   c->err.clear();
   check(c, hipSetDevice(c->device), "hipSetDevice");
-  if (c->arena) {
-    // no-allocation call: the error word and the statistics come first in
-    // the scratch (NoallocCall swaps them in); no events are created
-  } else {
-    if (!c->d_err) {
-      check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
-      check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
-    }
-    if (!c->ev_t0) check(c, hipEventCreate(&c->ev_t0), "event");
-    if (!c->ev_t1) check(c, hipEventCreate(&c->ev_t1), "event");
+  if (!c->d_err) {
+    check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
+    check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
   }
+  if (!c->ev_t0) check(c, hipEventCreate(&c->ev_t0), "event");
+  if (!c->ev_t1) check(c, hipEventCreate(&c->ev_t1), "event");
   fill_u32(c->d_err, 2, 0u, c->stream);  // [0] error bits, [1] eigensolver fallbacks
   memset(&c->stats, 0, sizeof c->stats);
   c->counts_ready = true;
@@ -383,23 +375,12 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
   const size_t budget = c->ws_budget;
-  const int nslot = (!c->timing && !c->arena && nz >= 4096) ? OAKG_PIPE_SLOTS : 1;
+  const int nslot = (!c->timing && nz >= 4096) ? c->tune_slots : 1;
   int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
   zc = std::min<int64_t>(zc, 1 << 20);
   if (nslot > 1) zc = std::min<int64_t>(zc, (nz + nslot - 1) / nslot);  // a chunk per slot
   zc = std::min<int64_t>(zc, nz);
   zc = (zc + 63) & ~(int64_t)63;  // slots start on 64-zone groups of the rotation log
-  if (c->arena) {
-    // chunks of what is left of the scratch (7 buffers of up to 255 bytes of
-    // alignment), in whole 64-zone groups of the rotation log
-    const size_t left = c->arena->left();
-    const int64_t fit = left > 8 * 256 ? (int64_t)((left - 8 * 256) / per_zone) : 0;
-    zc = std::min<int64_t>(zc, fit & ~(int64_t)63);
-    if (zc < 64) {
-      c->err = "the no-allocation scratch is too small for one group of 64 zones at this N";
-      throw Fail{OAKG_EARG};
-    }
-  }
   const int64_t zs = zc * nslot;
   double* mat = (double*)wsget(c, Workspace::MAT, sizeof(double) * (size_t)NP * NP * zs);
   double* ymat = (double*)wsget(c, Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zs);
@@ -414,18 +395,76 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     rlog = (char*)wsget(c, Workspace::RLOG, (size_t)16 * cap * zs);
     tst = (int32_t*)wsget(c, Workspace::TQL_STATUS, sizeof(int32_t) * zs);
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
-    fill_i32(sweeps_all, nz, 0, c->stream);
   }
-  hipStream_t S[OAKG_PIPE_SLOTS];
-  for (int i = 0; i < OAKG_PIPE_SLOTS; ++i) S[i] = c->stream;
+  // streams and events of the pipeline, created before any capture
   if (nslot > 1) {
     if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
-    check(c, hipEventRecord(c->ev_fork, S[0]), "event");
     for (int i = 1; i < nslot; ++i) {
       if (!c->aux[i - 1])
         check(c, hipStreamCreateWithFlags(&c->aux[i - 1], hipStreamNonBlocking), "stream");
       if (!c->ev_join[i - 1])
         check(c, hipEventCreateWithFlags(&c->ev_join[i - 1], hipEventDisableTiming), "event");
+    }
+  }
+  // HIP graph: the launches below depend only on these values; the same key
+  // replays the captured graph (on the context's capture stream, ordered
+  // after the caller's stream and joined back into it)
+  const bool use_graph = c->graph && !c->timing;
+  std::vector<char> key;
+  if (use_graph) {
+    struct Key {
+      ChunkArgs base;
+      int64_t nz, zc, cap;
+      int NP, nslot, ql, ql_zw;
+      const void* buf[8];
+      hipStream_t stream;
+    } kk;
+    memset(&kk, 0, sizeof kk);
+    kk.base = base;
+    kk.nz = nz;
+    kk.zc = zc;
+    kk.cap = cap;
+    kk.NP = NP;
+    kk.nslot = nslot;
+    kk.ql = ql;
+    kk.ql_zw = c->tune_ql_zw;
+    const void* bufs[8] = {mat, ymat, vec, dvec, evec, rlog, tst, sweeps_all};
+    memcpy(kk.buf, bufs, sizeof bufs);
+    kk.stream = c->stream;
+    key.assign((const char*)&kk, (const char*)&kk + sizeof kk);
+    if (!c->cap_stream)
+      check(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking), "stream");
+    if (!c->ev_cap_in) check(c, hipEventCreateWithFlags(&c->ev_cap_in, hipEventDisableTiming), "event");
+    if (!c->ev_cap_out) check(c, hipEventCreateWithFlags(&c->ev_cap_out, hipEventDisableTiming), "event");
+    check(c, hipEventRecord(c->ev_cap_in, c->stream), "event");
+    check(c, hipStreamWaitEvent(c->cap_stream, c->ev_cap_in, 0), "stream wait");
+    if (c->gexec && key == c->gkey) {
+      check(c, hipGraphLaunch(c->gexec, c->cap_stream), "hipGraphLaunch");
+      ++c->graph_replays;
+      check(c, hipEventRecord(c->ev_cap_out, c->cap_stream), "event");
+      check(c, hipStreamWaitEvent(c->stream, c->ev_cap_out, 0), "stream wait");
+      return;
+    }
+    check(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal), "capture");
+  }
+  hipStream_t s0 = use_graph ? c->cap_stream : c->stream;
+  // ends a capture left open by an error below (the stream stays usable)
+  struct CaptureGuard {
+    hipStream_t s;
+    bool on;
+    ~CaptureGuard() {
+      if (on) {
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      }
+    }
+  } guard{s0, use_graph};
+  if (ql) fill_i32(sweeps_all, nz, 0, s0);
+  hipStream_t S[OAKG_PIPE_SLOTS];
+  for (int i = 0; i < OAKG_PIPE_SLOTS; ++i) S[i] = s0;
+  if (nslot > 1) {
+    check(c, hipEventRecord(c->ev_fork, S[0]), "event");
+    for (int i = 1; i < nslot; ++i) {
       S[i] = c->aux[i - 1];
       check(c, hipStreamWaitEvent(S[i], c->ev_fork, 0), "stream wait");
     }
@@ -447,12 +486,29 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     A.rlog = ql ? (void*)(rlog + (size_t)16 * cap * zo) : nullptr;
     A.rlog_cap = cap;
     A.tql_status = ql ? tst + zo : nullptr;
+    A.ql_zw = c->tune_ql_zw;
     launch_chunk(NP, A, S[sl], c->timing ? &c->timer : nullptr);
     check(c, hipGetLastError(), "chunk launch");
   }
   for (int i = 1; i < nslot; ++i) {
     check(c, hipEventRecord(c->ev_join[i - 1], S[i]), "event");
     check(c, hipStreamWaitEvent(S[0], c->ev_join[i - 1], 0), "stream wait");
+  }
+  if (use_graph) {
+    guard.on = false;
+    hipGraph_t g = nullptr;
+    check(c, hipStreamEndCapture(c->cap_stream, &g), "end capture");
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    check(c, e, "hipGraphInstantiate");
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = ex;
+    c->gkey = key;
+    ++c->graph_captures;
+    check(c, hipGraphLaunch(c->gexec, c->cap_stream), "hipGraphLaunch");
+    check(c, hipEventRecord(c->ev_cap_out, c->cap_stream), "event");
+    check(c, hipStreamWaitEvent(c->stream, c->ev_cap_out, 0), "stream wait");
   }
 }
 
@@ -505,7 +561,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   if (ldSf < n || ldSa < n || (m > 0 && ldHSf < m)) arg_fail(c, "leading dimension too small");
   begin_call(c);
   if (c->timing) c->timer.mark(PH_PREP);
-  if (!c->arena) check(c, hipEventRecord(c->ev_t0, c->stream), "event");
+  check(c, hipEventRecord(c->ev_t0, c->stream), "event");
 
   oakg_zones zz{};
   if (csr) {
@@ -600,10 +656,10 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   A.maxcorr = HUGE_VAL;
   run_zones(c, NP, A, nz, sweeps);
   if (c->timing) c->timer.mark(PH_END);
-  if (!c->arena) check(c, hipEventRecord(c->ev_t1, c->stream), "event");
+  check(c, hipEventRecord(c->ev_t1, c->stream), "event");
   finish_call(c, nz, sel.ptr, sweeps);
   float ms = 0;
-  if (!c->arena) (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
+  (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
   collect_stats(c, nz, sel.ptr, ms);
   (void)total;
   if (!dev) {
@@ -1031,6 +1087,234 @@ int ensemble_report_impl(oakg_ctx* c, const oakg_obs* obs, const oakg_csr* H, co
   });
 }
 
+// genObservationOper + packSparseMatrix on the GPU (interp.hip): the box
+// trees and simplex tables are built on the host from the node coordinates
+// (copied back first when they are device arrays), everything per
+// observation runs on the device.
+int obs_operator_impl(oakg_ctx* c, int32_t nvar, const oakg_grid* grids, int64_t m,
+                      const int32_t* obs_var, const double* ox, const double* oy, const double* oz,
+                      const double* ot, int64_t* row_ptr, int32_t* col, double* val, int64_t cap,
+                      uint8_t* valid, int64_t* nnz, uint32_t flags) {
+  const bool dev = flags & OAKG_DEVICE_PTRS;
+  if (nvar < 0 || (nvar > 0 && !grids)) arg_fail(c, "grids required");
+  if (m < 0 || m > INT32_MAX) arg_fail(c, "m out of range");
+  if (!row_ptr || !valid || !nnz || (m > 0 && (!obs_var || !ox))) arg_fail(c, "row_ptr, valid, nnz, obs_var and ox are required");
+  if (cap > 0 && (!col || !val)) arg_fail(c, "col and val are required");
+  int nmax = 1;
+  std::vector<int64_t> nodes(nvar);
+  for (int v = 0; v < nvar; ++v) {
+    const oakg_grid& g = grids[v];
+    if (g.ndim < 1 || g.ndim > 4) arg_fail(c, "grid ndim must be 1..4");
+    int64_t tot = 1;
+    for (int d = 0; d < g.ndim; ++d) {
+      if (g.shape[d] < 2) {
+        c->err = "a grid dimension of one node makes every simplex degenerate (OAK's SVD branch): not supported";
+        throw Fail{OAKG_EUNSUP};
+      }
+      if (!g.coord[d]) arg_fail(c, "grid coordinates required");
+      tot *= g.shape[d];
+    }
+    if (tot > INT32_MAX) arg_fail(c, "grid above 2^31 nodes");
+    nodes[v] = tot;
+    nmax = std::max(nmax, (int)g.ndim);
+  }
+  const double* pos[4] = {ox, oy, oz, ot};
+  for (int v = 0; v < nvar; ++v)
+    for (int d = 0; d < grids[v].ndim; ++d)
+      if (m > 0 && !pos[d]) arg_fail(c, "observation coordinate missing for a grid's dimension");
+  begin_call(c);
+  const int stride = 1 << nmax;
+  // host side: trees and simplex tables, concatenated over the variables
+  std::vector<BoxDev> boxes;
+  std::vector<int64_t> box0(nvar), tet0(nvar);
+  std::vector<double> tets;
+  for (int v = 0; v < nvar; ++v) {
+    const oakg_grid& g = grids[v];
+    GridHost gh{};
+    gh.n = g.ndim;
+    std::vector<std::vector<double>> hc(g.ndim);
+    for (int d = 0; d < 4; ++d) gh.shape[d] = d < g.ndim ? g.shape[d] : 1;
+    for (int d = 0; d < g.ndim; ++d) {
+      if (dev) {
+        hc[d].resize(nodes[v]);
+        check(c, hipMemcpy(hc[d].data(), g.coord[d], sizeof(double) * nodes[v], hipMemcpyDeviceToHost),
+              "D2H grid");
+        gh.coord[d] = hc[d].data();
+      } else {
+        gh.coord[d] = g.coord[d];
+      }
+    }
+    // each variable's tree keeps its own box numbering; its launch gets the
+    // address of its first box
+    std::vector<BoxDev> b = build_box_tree(gh);
+    box0[v] = (int64_t)boxes.size();
+    boxes.insert(boxes.end(), b.begin(), b.end());
+    if ((int64_t)boxes.size() > INT32_MAX) arg_fail(c, "grid trees above 2^31 boxes");
+    std::vector<double> t = tet_table(g.ndim);
+    tet0[v] = (int64_t)tets.size();
+    tets.insert(tets.end(), t.begin(), t.end());
+  }
+  BoxDev* dbox = (BoxDev*)wsget(c, Workspace::OP_BOXES, sizeof(BoxDev) * std::max<size_t>(boxes.size(), 1));
+  double* dtet = (double*)wsget(c, Workspace::OP_TET, sizeof(double) * std::max<size_t>(tets.size(), 1));
+  if (!dbox || !dtet) check(c, hipErrorOutOfMemory, "workspace");
+  if (!boxes.empty())
+    check(c, hipMemcpyAsync(dbox, boxes.data(), sizeof(BoxDev) * boxes.size(), hipMemcpyHostToDevice,
+                            c->stream), "H2D boxes");
+  if (!tets.empty())
+    check(c, hipMemcpyAsync(dtet, tets.data(), sizeof(double) * tets.size(), hipMemcpyHostToDevice,
+                            c->stream), "H2D tetrahedra");
+  ObsPosDev o{};
+  o.var = stage(c, Workspace::OP_VAR, obs_var, m, dev);
+  o.x = stage(c, Workspace::OP_X, ox, m, dev);
+  o.y = stage(c, Workspace::OP_Y, oy, m, dev);
+  o.z = stage(c, Workspace::OP_Z, oz, m, dev);
+  o.t = stage(c, Workspace::OP_T, ot, m, dev);
+  const int64_t mm = std::max<int64_t>(m, 1);
+  int32_t* slotc = (int32_t*)wsget(c, Workspace::OP_SLOTC, sizeof(int32_t) * mm * stride);
+  double* slotw = (double*)wsget(c, Workspace::OP_SLOTW, sizeof(double) * mm * stride);
+  int64_t* cnt = (int64_t*)wsget(c, Workspace::OP_CNT, sizeof(int64_t) * (mm + 1));
+  int64_t* drp = dev ? row_ptr : (int64_t*)wsget(c, Workspace::OP_RP, sizeof(int64_t) * (mm + 1));
+  uint8_t* dvalid = dev ? valid : (uint8_t*)wsget(c, Workspace::OP_VALID, mm);
+  unsigned* status = (unsigned*)wsget(c, Workspace::OP_STATUS, sizeof(unsigned));
+  if (!slotc || !slotw || !cnt || !drp || !dvalid || !status) check(c, hipErrorOutOfMemory, "workspace");
+  fill_i32(slotc, m * stride, -1, c->stream);
+  fill_i64(cnt, m + 1, 0, c->stream);
+  fill_u32(status, 1, 0u, c->stream);
+  fill_u8(dvalid, m, 0, c->stream);
+  // host grids: staged one after the other into three buffers of the call
+  double* gc = nullptr;
+  uint8_t* gm = nullptr;
+  int64_t* gs = nullptr;
+  if (!dev) {
+    int64_t nc = 0, nm = 0, ns = 0;
+    for (int v = 0; v < nvar; ++v) {
+      nc += nodes[v] * grids[v].ndim;
+      if (grids[v].masked) nm += nodes[v];
+      if (grids[v].state_index) ns += nodes[v];
+    }
+    gc = (double*)wsget(c, Workspace::OP_GC, sizeof(double) * std::max<int64_t>(nc, 1));
+    gm = (uint8_t*)wsget(c, Workspace::OP_GM, std::max<int64_t>(nm, 1));
+    gs = (int64_t*)wsget(c, Workspace::OP_GS, sizeof(int64_t) * std::max<int64_t>(ns, 1));
+    if (!gc || !gm || !gs) check(c, hipErrorOutOfMemory, "workspace");
+  }
+  // per variable: its grid arrays on the device and one launch over the observations
+  for (int v = 0; v < nvar; ++v) {
+    const oakg_grid& g = grids[v];
+    GridDev gd{};
+    gd.n = g.ndim;
+    for (int d = 0; d < 4; ++d) gd.shape[d] = d < g.ndim ? g.shape[d] : 1;
+    if (dev) {
+      for (int d = 0; d < g.ndim; ++d) gd.coord[d] = g.coord[d];
+      gd.mask = g.masked;
+      gd.state = g.state_index;
+    } else {
+      for (int d = 0; d < g.ndim; ++d) {
+        check(c, hipMemcpyAsync(gc, g.coord[d], sizeof(double) * nodes[v], hipMemcpyHostToDevice,
+                                c->stream), "H2D grid");
+        gd.coord[d] = gc;
+        gc += nodes[v];
+      }
+      if (g.masked) {
+        check(c, hipMemcpyAsync(gm, g.masked, nodes[v], hipMemcpyHostToDevice, c->stream), "H2D mask");
+        gd.mask = gm;
+        gm += nodes[v];
+      }
+      if (g.state_index) {
+        check(c, hipMemcpyAsync(gs, g.state_index, sizeof(int64_t) * nodes[v], hipMemcpyHostToDevice,
+                                c->stream), "H2D state index");
+        gd.state = gs;
+        gs += nodes[v];
+      }
+    }
+    if (m > 0)
+      launch_obs_interp(g.ndim, m, v, gd, dbox + box0[v], dtet + tet0[v], tet_count(g.ndim), o, stride,
+                        slotc, slotw, cnt, dvalid, status, c->stream);
+    check(c, hipGetLastError(), "interp launch");
+  }
+  check(c, scan_counts(c->ws, cnt, drp, m, c->stream), "scan");
+  int64_t tot = 0;
+  unsigned hst = 0;
+  check(c, hipMemcpyAsync(&tot, drp + m, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "D2H nnz");
+  check(c, hipMemcpyAsync(&hst, status, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "D2H status");
+  check(c, hipStreamSynchronize(c->stream), "sync");
+  if (hst & 1u) {
+    c->err = "an observation's cell has degenerate simplices (|det| <= 1e-8, OAK's SVD branch): not supported";
+    return OAKG_EUNSUP;
+  }
+  if (hst & 2u) {
+    c->err = "grid search deeper than 48 levels";
+    return OAKG_EUNSUP;
+  }
+  *nnz = tot;
+  if (tot > cap) {
+    c->err = "cap below the number of entries of H (returned in *nnz)";
+    return OAKG_EARG;
+  }
+  int32_t* dcol = dev ? col : (int32_t*)wsget(c, Workspace::OP_COL, sizeof(int32_t) * std::max<int64_t>(tot, 1));
+  double* dval = dev ? val : (double*)wsget(c, Workspace::OP_VAL, sizeof(double) * std::max<int64_t>(tot, 1));
+  if (!dcol || !dval) check(c, hipErrorOutOfMemory, "workspace");
+  launch_obs_compact(m, stride, drp, slotc, slotw, dcol, dval, c->stream);
+  check(c, hipGetLastError(), "compact launch");
+  check(c, hipStreamSynchronize(c->stream), "sync");
+  if (!dev) {
+    check(c, hipMemcpy(row_ptr, drp, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost), "D2H row_ptr");
+    if (m > 0) check(c, hipMemcpy(valid, dvalid, (size_t)m, hipMemcpyDeviceToHost), "D2H valid");
+    if (tot > 0) {
+      check(c, hipMemcpy(col, dcol, sizeof(int32_t) * tot, hipMemcpyDeviceToHost), "D2H col");
+      check(c, hipMemcpy(val, dval, sizeof(double) * tot, hipMemcpyDeviceToHost), "D2H val");
+    }
+  }
+  return OAKG_OK;
+}
+
+// Gather (load) or scatter (save) of ensemble rows between the model's full
+// field vectors and the zone-permuted state.
+int permute_rows_impl(oakg_ctx* c, bool gather, int64_t n, int32_t N, const int64_t* idx,
+                      int64_t full_len, const void* in, int64_t ldi, void* out, int64_t ldo,
+                      int32_t elem, uint32_t flags) {
+  const bool dev = flags & OAKG_DEVICE_PTRS;
+  if (elem != 4 && elem != 8) arg_fail(c, "elem_size must be 4 or 8");
+  if (n < 0 || N < 0 || full_len < 0) arg_fail(c, "negative size");
+  if (n > 0 && N > 0 && (!idx || !in || !out)) arg_fail(c, "idx, in and out are required");
+  const int64_t rin = gather ? full_len : n, rout = gather ? n : full_len;
+  if (N > 1 && (ldi < rin || ldo < rout)) arg_fail(c, "leading dimension too small");
+  if (N > 65535) arg_fail(c, "N above 65535");
+  if (!dev)
+    for (int64_t l = 0; l < n; ++l)
+      if (idx[l] < 0 || idx[l] >= full_len) arg_fail(c, "idx out of [0, full_len)");
+  begin_call(c);
+  if (n == 0 || N == 0) return OAKG_OK;
+  const int64_t li = N > 1 ? ldi : rin, lo = N > 1 ? ldo : rout;
+  const int64_t* didx = stage(c, Workspace::PR_IDX, idx, n, dev);
+  const void* din = in;
+  void* dout = out;
+  int64_t dli = li, dlo = lo;
+  if (!dev) {
+    char* bi = (char*)wsget(c, Workspace::PR_IN, (size_t)elem * rin * N);
+    char* bo = (char*)wsget(c, Workspace::PR_OUT, (size_t)elem * rout * N);
+    if (!bi || !bo) check(c, hipErrorOutOfMemory, "workspace");
+    for (int k = 0; k < N; ++k) {
+      check(c, hipMemcpyAsync(bi + (size_t)elem * rin * k, (const char*)in + (size_t)elem * li * k,
+                              (size_t)elem * rin, hipMemcpyHostToDevice, c->stream), "H2D");
+      if (!gather)  // the scatter leaves untouched rows as they are
+        check(c, hipMemcpyAsync(bo + (size_t)elem * rout * k, (const char*)out + (size_t)elem * lo * k,
+                                (size_t)elem * rout, hipMemcpyHostToDevice, c->stream), "H2D");
+    }
+    din = bi;
+    dout = bo;
+    dli = rin;
+    dlo = rout;
+  }
+  launch_permute_rows(gather, elem, n, N, didx, din, dli, dout, dlo, full_len, c->d_err, c->stream);
+  check(c, hipGetLastError(), "permute launch");
+  finish_call(c, 0, nullptr, nullptr);
+  if (!dev)
+    for (int k = 0; k < N; ++k)
+      check(c, hipMemcpy((char*)out + (size_t)elem * lo * k, (const char*)dout + (size_t)elem * rout * k,
+                         (size_t)elem * rout, hipMemcpyDeviceToHost), "D2H");
+  return OAKG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1076,6 +1360,13 @@ int oakg_destroy(oakg_ctx* c) {
     if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->cap_stream) {
+    (void)hipStreamSynchronize(c->cap_stream);
+    (void)hipStreamDestroy(c->cap_stream);
+  }
+  if (c->ev_cap_in) (void)hipEventDestroy(c->ev_cap_in);
+  if (c->ev_cap_out) (void)hipEventDestroy(c->ev_cap_out);
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
   if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
   if (c->d_err) (void)hipFree(c->d_err);
@@ -1107,6 +1398,47 @@ int oakg_set_eigensolver(oakg_ctx* c, int method) {
   }
   c->eig_method = method;
   return OAKG_OK;
+}
+
+int oakg_set_tuning(oakg_ctx* c, int key, int value) {
+  if (!c) return OAKG_EARG;
+  switch (key) {
+    case OAKG_TUNE_SLOTS:
+      if (value < 1 || value > OAKG_PIPE_SLOTS) break;
+      c->tune_slots = value;
+      return OAKG_OK;
+    case OAKG_TUNE_QL_ZONES:
+      if (value != 8 && value != 16 && value != 32) break;
+      c->tune_ql_zw = value;
+      return OAKG_OK;
+    case OAKG_TUNE_GRAPH:
+      if (value != 0 && value != 1) break;
+      c->graph = value != 0;
+      if (!c->graph && c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+        c->gkey.clear();
+      }
+      return OAKG_OK;
+    default:
+      break;
+  }
+  c->err = "oakg_set_tuning: unknown key or value out of range";
+  return OAKG_EARG;
+}
+
+int oakg_get_tuning(oakg_ctx* c, int key, int64_t* value) {
+  if (!c || !value) return OAKG_EARG;
+  switch (key) {
+    case OAKG_TUNE_SLOTS: *value = c->tune_slots; return OAKG_OK;
+    case OAKG_TUNE_QL_ZONES: *value = c->tune_ql_zw; return OAKG_OK;
+    case OAKG_TUNE_GRAPH: *value = c->graph ? 1 : 0; return OAKG_OK;
+    case OAKG_TUNE_GRAPH_REPLAYS: *value = c->graph_replays; return OAKG_OK;
+    case OAKG_TUNE_GRAPH_CAPTURES: *value = c->graph_captures; return OAKG_OK;
+    default: break;
+  }
+  c->err = "oakg_get_tuning: unknown key";
+  return OAKG_EARG;
 }
 
 int oakg_get_stats(oakg_ctx* c, oakg_stats* out) {
@@ -1227,6 +1559,32 @@ int oakg_local_nll(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, in
     if (!zones || !obs) arg_fail(c, "zones and obs are required");
     return local_nll_impl(c, zones, obs, N, Hxf, HSf, ldHSf, inflation, nll, grad_log_sigma,
                           grad_corr_len, grad_inflation, nll_zone, flags);
+  });
+}
+
+int oakg_obs_operator(oakg_ctx* c, int32_t nvar, const oakg_grid* grids, int64_t m,
+                      const int32_t* obs_var, const double* ox, const double* oy,
+                      const double* oz, const double* ot, int64_t* row_ptr, int32_t* col,
+                      double* val, int64_t cap, uint8_t* valid, int64_t* nnz, uint32_t flags) {
+  return guarded(c, [&] {
+    return obs_operator_impl(c, nvar, grids, m, obs_var, ox, oy, oz, ot, row_ptr, col, val, cap,
+                             valid, nnz, flags);
+  });
+}
+
+int oakg_gather_rows(oakg_ctx* c, int64_t n, int32_t N, const int64_t* idx, int64_t full_len,
+                     const void* in, int64_t ldi, void* out, int64_t ldo, int32_t elem_size,
+                     uint32_t flags) {
+  return guarded(c, [&] {
+    return permute_rows_impl(c, true, n, N, idx, full_len, in, ldi, out, ldo, elem_size, flags);
+  });
+}
+
+int oakg_scatter_rows(oakg_ctx* c, int64_t n, int32_t N, const int64_t* idx, int64_t full_len,
+                      const void* in, int64_t ldi, void* out, int64_t ldo, int32_t elem_size,
+                      uint32_t flags) {
+  return guarded(c, [&] {
+    return permute_rows_impl(c, false, n, N, idx, full_len, in, ldi, out, ldo, elem_size, flags);
   });
 }
 

@@ -48,7 +48,7 @@ class Workspace {
     // staged copies of host inputs and outputs
     Z_PTR, Z_POS, Z_CORR, Z_MAXLEN, O_X, O_Y, O_Z, O_T, O_YO, O_RMSE2, O_EXCL, IN_S, IN_HSF,
     IN_XF, IN_HXF, H_RP, H_COL, H_VAL, H_SHIFT, OUT_XA, OUT_S, OUT_AMPL, U_SEL_PTR, U_SEL_IDX,
-    U_SEL_W, MAXCORR, REP_PIECES, REP_PBEG, REP_PART, REP_STATS, REP_D, REP_HS, NSLOTS
+    U_SEL_W, MAXCORR, OP_BOXES, OP_TET, OP_VAR, OP_X, OP_Y, OP_Z, OP_T, OP_SLOTC, OP_SLOTW, OP_CNT, OP_RP, OP_VALID, OP_COL, OP_VAL, OP_STATUS, OP_GC, OP_GM, OP_GS, PR_IDX, PR_IN, PR_OUT, REP_PIECES, REP_PBEG, REP_PART, REP_STATS, REP_D, REP_HS, NSLOTS
   };
   Workspace() : ptr_(NSLOTS, nullptr), cap_(NSLOTS, 0) {}
   ~Workspace() { release(); }
@@ -87,5 +87,9 @@ hipError_t scan_counts(Workspace& ws, const int64_t* counts, int64_t* sel_ptr, i
 void fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t st);
 void fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+void launch_permute_rows(bool gather, int elem, int64_t n, int N, const int64_t* idx,
+                         const void* in, int64_t ldi, void* out, int64_t ldo, int64_t lim,
+                         unsigned* err, hipStream_t st);
 
 }  // namespace oakg

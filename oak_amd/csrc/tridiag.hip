@@ -1264,9 +1264,18 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
                      A.dvec, A.evec, A.nzc);
   if constexpr (NP == 128)
     hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.ymat);
-  constexpr int ZW = 16;
-  hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0,
-                     A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
+  // zones per QL wave: a tuning knob (the rotation log layout groups 64
+  // zones whatever the wave width)
+#define OAK_TQL(ZW)                                                                               \
+  hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0, \
+                     A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err)
+  if (A.ql_zw == 8)
+    OAK_TQL(8);
+  else if (A.ql_zw == 32)
+    OAK_TQL(32);
+  else
+    OAK_TQL(16);
+#undef OAK_TQL
   if constexpr (NP == 128) {  // the epilogue runs at the end of the apply kernel
     hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
                        A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,

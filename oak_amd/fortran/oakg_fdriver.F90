@@ -30,9 +30,10 @@ program oakg_fdriver
   use oakg_rrsqrt
   use fdriver_geom
   implicit none
-  integer :: zoneSize(nzone), i, k, l, info, info2, u, n
+  integer :: zoneSize(nzone), i, k, l, info, info2, info3, u, n
   real(c_double), allocatable, target :: xf(:), Hxf(:), yo(:), Sf(:,:), HSf(:,:), Rd(:), xa(:), Sa(:,:)
-  real(c_double), allocatable, target :: xa2(:), Sa2(:,:)
+  real(c_double), allocatable, target :: xa2(:), Sa2(:,:), xa3(:), Sa3(:,:)
+  real(c_double), allocatable, target :: zpos(:,:), cl(:), ml(:), ox(:), oy(:)
   character(len=1024) :: fname
   logical, allocatable :: excl(:)
 
@@ -40,7 +41,8 @@ program oakg_fdriver
   n = nzone * zsize
   zoneSize = zsize
   allocate(xf(n), Hxf(nobs), yo(nobs), Sf(n, nmem), HSf(nobs, nmem), Rd(nobs), xa(n), Sa(n, nmem))
-  allocate(xa2(n), Sa2(n, nmem))
+  allocate(xa2(n), Sa2(n, nmem), xa3(n), Sa3(n, nmem))
+  allocate(zpos(4, nzone), cl(nzone), ml(nzone), ox(nobs), oy(nobs))
   do i = 1, n
     xf(i) = sin(0.3d0 * i)
     do k = 1, nmem
@@ -66,6 +68,26 @@ program oakg_fdriver
     call locAnalysis_gpu(zoneSize, select_gauss, xf, Hxf, yo, Sf, HSf, Rd, xa2, Sa2, &
          localise_obs=.false., info=info2)
   print '(a,i0)', 'locAnalysis_gpu (localise_obs = .false.) info = ', info2
+  ! geometry mode: the same selection (first element of zone i at
+  ! x = 0.5 ((i-1) zsize + 1) + 3, observation l at x = 1.25 l, Cartesian,
+  ! L = 2, cutoff 2) evaluated on the GPU
+  zpos = 0
+  do i = 1, nzone
+    zpos(1, i) = 0.5d0 * dble((i - 1) * zsize + 1) + 3.0d0
+  end do
+  cl = 2.0d0
+  ml = 2.0d0
+  do l = 1, nobs
+    ox(l) = 1.25d0 * dble(l)
+  end do
+  oy = 0
+  xa3 = 0
+  Sa3 = 0
+  info3 = info2
+  if (info2 == 0) &
+    call locAnalysis_geo_gpu(zoneSize, zpos, cl, ml, 0, 1, ox, oy, oy, oy, xf, Hxf, yo, Sf, HSf, &
+         Rd, xa3, Sa3, info=info3)
+  print '(a,i0)', 'locAnalysis_geo_gpu info = ', info3
   ! the forecast report of Assim (assimilation.F90:3389), the last
   ! observation excluded
   allocate(excl(nobs))
@@ -75,10 +97,12 @@ program oakg_fdriver
     call report_gpu(6, 'forecast.', 55000.5d0, count(.not. excl), Rd, HSf, yo - Hxf, excl)
   if (len_trim(fname) > 0) then
     open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
-    write(u) int(info, 8), int(nzone, 8), int(zsize, 8), int(nmem, 8), int(nobs, 8), int(info2, 8)
-    write(u) xf, Hxf, yo, Sf, HSf, Rd, xa, Sa, xa2, Sa2
+    write(u) int(info, 8), int(nzone, 8), int(zsize, 8), int(nmem, 8), int(nobs, 8), int(info2, 8), &
+         int(info3, 8)
+    write(u) xf, Hxf, yo, Sf, HSf, Rd, xa, Sa, xa2, Sa2, xa3, Sa3
     close(u)
   end if
   if (info /= 0) call exit(info)
   if (info2 /= 0) call exit(info2)
+  if (info3 /= 0) call exit(info3)
 end program oakg_fdriver

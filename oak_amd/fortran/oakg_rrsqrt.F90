@@ -16,7 +16,7 @@ module oakg_rrsqrt
   use oakg
   implicit none
   private
-  public :: locAnalysis_gpu, report_gpu, oakg_rrsqrt_context
+  public :: locAnalysis_gpu, locAnalysis_geo_gpu, report_gpu, oakg_rrsqrt_context
 
   type(c_ptr), save :: ctx = c_null_ptr
 
@@ -137,6 +137,86 @@ contains
       nn = cap
     end subroutine grow
   end subroutine locAnalysis_gpu
+
+  ! locAnalysis with OAK's own selectObservations (assimilation.F90:3683-3771)
+  ! evaluated on the GPU for every zone at once, instead of a callback per
+  ! zone on the host: the drop-in for the call in Assim
+  ! (assimilation.F90:3235-3236).  The assimilation module state that OAK's
+  ! selectObservations reads is passed flattened:
+  !   zpos(4, nzones)  x, y, z, t of each zone's first element, i.e. of
+  !                    zoneIndex(startIndex(zi)) through ind2submv and getCoord
+  !                    (:3713-3740)
+  !   corrLen, maxLen  hCorrLengthToObs / hMaxCorrLengthToObs of that element
+  !                    (:414-415, 3741-3742), one per zone
+  !   metric, loctype  metrictype (0 Cartesian, 1 spherical, 2 approximate)
+  !                    and loctype (1 horizontal, 2 |dz|, 3 |dt|)
+  !   obsX, obsY, obsZ, obsT  obsGridX/Y/Z/T (:228, 3159-3167)
+  ! The other arguments are those of locAnalysis_gpu.
+  subroutine locAnalysis_geo_gpu(zoneSize, zpos, corrLen, maxLen, metric, loctype, obsX, obsY, &
+       obsZ, obsT, xf, Hxf, yo, Sf, HSf, Rdiag, xa, Sa, amplitudes, localise_obs, info)
+    integer, intent(in) :: zoneSize(:), metric, loctype
+    real(c_double), intent(in), target, contiguous :: zpos(:,:), corrLen(:), maxLen(:)
+    real(c_double), intent(in), target, contiguous :: obsX(:), obsY(:), obsZ(:), obsT(:)
+    real(c_double), intent(in), target, contiguous :: xf(:), Hxf(:), yo(:), Sf(:,:), HSf(:,:), Rdiag(:)
+    real(c_double), intent(out), target, contiguous :: xa(:), Sa(:,:)
+    real(c_double), intent(out), target, optional, contiguous :: amplitudes(:,:)
+    logical, intent(in), optional :: localise_obs
+    integer, intent(out), optional :: info
+
+    integer(c_int64_t), allocatable, target :: zone_ptr(:)
+    type(oakg_zones) :: zones
+    type(oakg_obs) :: obs
+    type(c_ptr) :: c, pampl
+    integer(c_int) :: rc
+    integer :: nz, m, n, iz
+    integer(c_int32_t) :: loc
+
+!$omp master
+    nz = size(zoneSize)
+    m = size(yo)
+    n = size(xf)
+    loc = 1
+    if (present(localise_obs)) then
+      if (.not. localise_obs) loc = 0
+    end if
+    allocate(zone_ptr(nz + 1))
+    zone_ptr(1) = 0
+    do iz = 1, nz
+      zone_ptr(iz + 1) = zone_ptr(iz) + zoneSize(iz)
+    end do
+    zones%nzones = nz
+    zones%zone_ptr = c_loc(zone_ptr)
+    zones%zpos = c_loc(zpos)
+    zones%corr_len = c_loc(corrLen)
+    zones%max_len = c_loc(maxLen)
+    zones%metric = int(metric, c_int32_t)
+    zones%loctype = int(loctype, c_int32_t)
+    obs%m = m
+    obs%x = c_loc(obsX)
+    obs%y = c_loc(obsY)
+    obs%z = c_loc(obsZ)
+    obs%t = c_loc(obsT)
+    obs%yo = c_loc(yo)
+    obs%rmse2 = c_loc(Rdiag)
+    pampl = c_null_ptr
+    if (present(amplitudes)) pampl = c_loc(amplitudes)
+    c = oakg_rrsqrt_context()
+    if (.not. c_associated(c)) then
+      rc = OAKG_EHIP
+    else
+      rc = oakg_local_analysis(c, zones, obs, int(size(Sf, 2), c_int32_t), c_loc(xf), c_loc(Hxf), &
+           c_loc(Sf), int(n, c_int64_t), c_loc(HSf), int(m, c_int64_t), c_loc(xa), c_loc(Sa), &
+           int(n, c_int64_t), pampl, loc, 0_c_int32_t)
+    end if
+    if (present(info)) then
+      info = rc
+    else if (rc /= OAKG_OK) then
+      if (c_associated(c)) write(0, *) 'locAnalysis_geo_gpu: ', oakg_error_string(c)
+      error stop 1
+    end if
+!$omp end master
+!$omp barrier
+  end subroutine locAnalysis_geo_gpu
 
   ! report (assimilation.F90:3584-3628) with the statistics computed on the
   ! GPU: the same arguments, with R given by its diagonal Rdiag (R%D), and the

@@ -328,3 +328,108 @@ def partition(part):
     izi = np.zeros(n, dtype=np.int64)
     nz = lib().oko_partition(C.c_int64(n), _ptr(part), _ptr(zs), _ptr(zi), _ptr(izi))
     return zs[:nz].copy(), zi, izi
+
+
+# --- observation operator from coordinates (oak_oracle_interp.c) ---
+
+def _factorial(n):
+    f = 1
+    for i in range(2, n + 1):
+        f *= i
+    return f
+
+
+def tetrahedra(n):
+    """The hyper-tetrahedral split of the unit cube (ndgrid.F90:357-435):
+    array (2^n, n+1, n! 2^(n-1)), column-major as OAK's g%tetrahedron."""
+    nb = _factorial(n) * 2 ** (n - 1)
+    tc = np.zeros((2 ** n, n + 1, nb), order="F")
+    if lib().oko_tetrahedra(C.c_int(n), _ptr(tc)) != 0:
+        raise OracleError("tetrahedra: n out of range")
+    return tc
+
+
+def _grid_arrays(coords):
+    """coords: list of n arrays of the grid's shape (Fortran order: the first
+    index fastest), the node coordinates per dimension."""
+    n = len(coords)
+    shape = np.ones(4, dtype=np.int64)
+    shape[:n] = coords[0].shape
+    flat = [np.ascontiguousarray(np.asarray(c, dtype=np.float64).ravel(order="F"))
+            for c in coords]
+    return n, shape, flat
+
+
+def cinterp(coords, xi, mask=None):
+    """cinterp (ndgrid.F90:1183-1257) at one point: (node linear indices
+    (0-based, first index fastest), weights) of the 2^n cell corners, or
+    None when xi is outside the grid or next to a masked node."""
+    n, shape, flat = _grid_arrays(coords)
+    cp = (C.c_void_p * 4)(*([a.ctypes.data for a in flat] + [None] * (4 - n)))
+    mk = None if mask is None else np.ascontiguousarray(np.asarray(mask, np.uint8).ravel(order="F"))
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    cidx = np.zeros(16, dtype=np.int64)
+    coeff = np.zeros(16)
+    L = lib()
+    L.oko_cinterp.restype = C.c_int
+    r = L.oko_cinterp(C.c_int(n), _ptr(shape), cp, _ptr(mk), _ptr(xi), _ptr(cidx), _ptr(coeff))
+    if r < 0:
+        raise OracleError("cinterp rc=%d" % -r)
+    if r == 0:
+        return None
+    return cidx[:r].copy(), coeff[:r].copy()
+
+
+def obs_operator(grids, obs_var, obs_pos):
+    """genObservationOper + packSparseMatrix (assimilation.F90:2471-2656,
+    4132-4208) for one model variable per observation.
+
+    grids: list of dicts with 'coords' (n arrays of the grid shape, Fortran
+    order), optional 'mask' (True = land) and 'state' (node -> 0-based state
+    column, -1 = not in the state; default: the node index).
+    obs_var: [m] int (-1 = unknown variable); obs_pos: list of up to 4 arrays
+    [m] (x, y, z, t).  Returns (row_ptr, col, val, valid) with H in CSR by
+    observation."""
+    nv = len(grids)
+    ndim = np.zeros(max(nv, 1), dtype=np.int32)
+    shape = np.ones((max(nv, 1), 4), dtype=np.int64)
+    keep = []
+    cptr = (C.c_void_p * (4 * max(nv, 1)))()
+    mptr = (C.c_void_p * max(nv, 1))()
+    sptr = (C.c_void_p * max(nv, 1))()
+    twon_max = 1
+    for v, g in enumerate(grids):
+        n, sh, flat = _grid_arrays(g["coords"])
+        ndim[v] = n
+        shape[v] = sh
+        twon_max = max(twon_max, 2 ** n)
+        keep += flat
+        for d in range(n):
+            cptr[4 * v + d] = flat[d].ctypes.data
+        if g.get("mask") is not None:
+            mk = np.ascontiguousarray(np.asarray(g["mask"], np.uint8).ravel(order="F"))
+            keep.append(mk)
+            mptr[v] = mk.ctypes.data
+        if g.get("state") is not None:
+            st = np.ascontiguousarray(np.asarray(g["state"], np.int64).ravel(order="F"))
+            keep.append(st)
+            sptr[v] = st.ctypes.data
+    obs_var = np.ascontiguousarray(obs_var, dtype=np.int32)
+    m = len(obs_var)
+    pos = [np.ascontiguousarray(p, dtype=np.float64) for p in obs_pos]
+    pp = [C.c_void_p(p.ctypes.data) for p in pos] + [None] * (4 - len(pos))
+    cap = max(1, m * twon_max)
+    row_ptr = np.zeros(m + 1, dtype=np.int64)
+    col = np.zeros(cap, dtype=np.int32)
+    val = np.zeros(cap)
+    valid = np.zeros(max(m, 1), dtype=np.uint8)
+    L = lib()
+    L.oko_obs_operator.restype = C.c_int
+    rc = L.oko_obs_operator(C.c_int(nv), _ptr(ndim), _ptr(np.ascontiguousarray(shape.ravel())),
+                            cptr, mptr, sptr, C.c_int64(m), _ptr(obs_var), pp[0], pp[1], pp[2],
+                            pp[3], C.c_int64(cap), _ptr(row_ptr), _ptr(col), _ptr(val),
+                            _ptr(valid))
+    if rc:
+        raise OracleError("obs_operator rc=%d" % rc)
+    nz = int(row_ptr[-1])
+    return row_ptr, col[:nz].copy(), val[:nz].copy(), valid[:m].astype(bool)

@@ -13,19 +13,19 @@ DRIVER = os.path.join(ROOT, "oak_amd", "fortran", "oakg_fdriver")
 
 def read(path):
     raw = open(path, "rb").read()
-    hdr = np.frombuffer(raw[:48], dtype=np.int64)
-    info, nzone, zsize, N, m, info2 = (int(v) for v in hdr)
+    hdr = np.frombuffer(raw[:56], dtype=np.int64)
+    info, nzone, zsize, N, m, info2, info3 = (int(v) for v in hdr)
     n = nzone * zsize
-    v = np.frombuffer(raw[48:], dtype=np.float64)
+    v = np.frombuffer(raw[56:], dtype=np.float64)
     sizes = [("xf", n), ("Hxf", m), ("yo", m), ("Sf", n * N), ("HSf", m * N), ("Rd", m),
-             ("xa", n), ("Sa", n * N), ("xa2", n), ("Sa2", n * N)]
+             ("xa", n), ("Sa", n * N), ("xa2", n), ("Sa2", n * N), ("xa3", n), ("Sa3", n * N)]
     out, o = {}, 0
     for k, s in sizes:
         out[k] = v[o:o + s]
         o += s
-    for k, r in (("Sf", n), ("HSf", m), ("Sa", n), ("Sa2", n)):
+    for k, r in (("Sf", n), ("HSf", m), ("Sa", n), ("Sa2", n), ("Sa3", n)):
         out[k] = out[k].reshape((r, N), order="F")
-    out.update(info=info, info2=info2, nzone=nzone, zsize=zsize, N=N, m=m, n=n)
+    out.update(info=info, info2=info2, info3=info3, nzone=nzone, zsize=zsize, N=N, m=m, n=n)
     return out
 
 

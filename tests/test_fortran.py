@@ -19,4 +19,5 @@ def test_fortran_driver_without_gpu(tmp_path):
     assert r.returncode == 3, r.stdout + r.stderr
     assert "info = 3" in r.stdout
     d = read(out)
-    assert d["info"] == 3 and d["info2"] == 3 and d["n"] == 20 and d["N"] == 6 and d["m"] == 8
+    assert d["info"] == 3 and d["info2"] == 3 and d["info3"] == 3
+    assert d["n"] == 20 and d["N"] == 6 and d["m"] == 8

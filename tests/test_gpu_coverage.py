@@ -120,11 +120,14 @@ def test_selection_metrics_and_loctypes_set_for_set(metric, loctype):
         cut = 2.0 * L
     elif loctype == 2:
         L = np.full(nz, 60.0); cut = np.full(nz, 120.0)
-        # an observation exactly at the cutoff of zone 0 (inclusive test)
-        oz[7] = zpos[0, 2] + cut[0]
+        # an observation exactly at the cutoff of zone 0 (inclusive test;
+        # values whose difference is exact in binary)
+        zpos[0, 2] = 100.0
+        oz[7] = 220.0
     else:
         L = np.full(nz, 3.0); cut = np.full(nz, 6.0)
-        ot[7] = zpos[0, 3] - cut[0]
+        zpos[0, 3] = 10.0
+        ot[7] = 4.0
     geo = A.Geometry(np.arange(nz + 1), zpos, L, cut, ox, oy, oz, ot, metric=metric,
                      loctype=loctype)
     yo = g.standard_normal(m); r2 = np.full(m, 0.3)

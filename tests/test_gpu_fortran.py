@@ -18,8 +18,8 @@ def test_fortran_locanalysis_vs_oracle(tmp_path):
     r = subprocess.run([DRIVER, str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     d = read(out)
-    assert d["info"] == 0 and d["info2"] == 0
-    for loc, kx, ks in ((True, "xa", "Sa"), (False, "xa2", "Sa2")):
+    assert d["info"] == 0 and d["info2"] == 0 and d["info3"] == 0
+    for loc, kx, ks in ((True, "xa", "Sa"), (False, "xa2", "Sa2"), (True, "xa3", "Sa3")):
         xa, Sa = oracle_analysis(d, localise_obs=loc)
         dx = np.abs(d[kx] - xa).max() / np.abs(xa - d["xf"]).max()
         ds = np.abs(d[ks] - Sa).max() / np.abs(Sa).max()
@@ -32,6 +32,11 @@ def test_fortran_locanalysis_vs_oracle(tmp_path):
     # index (the mistake this pins) gives a different analysis
     xz, _ = oracle_analysis(d, zone_number=True)
     assert np.abs(d["xa"] - xz).max() > 1e-3 * np.abs(xz - d["xf"]).max()
+    # geometry mode (selection on the GPU) = callback mode: the same sets,
+    # weights equal up to the last bit of exp (device vs host libm)
+    dx = np.abs(d["xa3"] - d["xa"]).max() / np.abs(d["xa"] - d["xf"]).max()
+    ds = np.abs(d["Sa3"] - d["Sa"]).max() / np.abs(d["Sa"]).max()
+    assert dx < 1e-13 and ds < 1e-13, (dx, ds)
 
 
 def test_fortran_report_vs_oracle(tmp_path):

@@ -1,0 +1,85 @@
+"""The zone loop captured in a HIP graph and replayed (OAKG_TUNE_GRAPH), and
+the tuning knobs (chunks in flight, zones per QL wave): every replay and
+every setting gives the bits of the eager, default run; a replay reads the
+current contents of the inputs; eager work between the replays does not
+disturb them.  Needs an MI355X."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def problem():
+    from oak_amd import synth
+    from oak_amd import assimilation as A
+    P = synth.column_ocean(64, 64, 1, 40, 3000, 100, seed_cfg=700, hlen=4, vlen=1, L=3.0,
+                           cutoff=9.0)
+    assert P.nzones >= 4096  # several zone chunks on several streams
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt)
+    g = A.Geometry(t(P.zone_ptr, torch.int64), t(P.zpos), t(P.corr_len), t(P.max_len), t(P.ox),
+                   t(P.oy))
+    rp, co, va = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    H = (t(rp, torch.int64), t(co, torch.int32), t(va))
+    return P, g, H, t(P.Hshift), t(P.yo), t(P.rmse2), t(P.E.T).t()
+
+
+def _run(ctx, pr, yo=None, out=None):
+    from oak_amd import assimilation as A
+    P, g, H, hs, yo0, r2, E = pr
+    Ea = out if out is not None else torch.empty((E.shape[1], E.shape[0]), dtype=torch.float64,
+                                                 device=E.device).t()
+    A.ensemble_analysis(g, H, hs, yo0 if yo is None else yo, r2, E, ctx=ctx, out=Ea)
+    torch.cuda.synchronize()
+    return Ea.cpu().numpy()
+
+
+def test_graph_replays_match_eager(problem):
+    from oak_amd._lib import Context
+    ref_ctx = Context(0)
+    ref = _run(ref_ctx, problem)
+    yo2 = problem[4] + 0.05
+    ref2 = _run(ref_ctx, problem, yo=yo2)
+    ref_ctx.close()
+    ctx = Context(0)
+    ctx.set_tuning(graph=1)
+    E = problem[6]
+    out = torch.empty((E.shape[1], E.shape[0]), dtype=torch.float64, device=E.device).t()
+    junk = torch.randn(2048, 2048, device=E.device, dtype=torch.float64)
+    for rep in range(4):
+        got = _run(ctx, problem, out=out)
+        np.testing.assert_array_equal(got, ref)
+        junk = junk @ junk.T / 2048.0  # eager work between the replays
+        torch.cuda.synchronize()
+    tu = ctx.tuning()
+    assert tu["graph_captures"] == 1 and tu["graph_replays"] == 3, tu
+    # the same graph with new observation values (same buffers)
+    yo_buf = problem[4].clone()
+    yo_buf.copy_(yo2)
+    got2 = _run(ctx, problem, yo=yo_buf, out=out)
+    np.testing.assert_array_equal(got2, ref2)
+    ctx.close()
+
+
+@pytest.mark.parametrize("slots,ql", [(1, 8), (2, 32), (4, 16)])
+def test_tuning_knobs_same_bits(problem, slots, ql):
+    from oak_amd._lib import Context
+    ref_ctx = Context(0)
+    ref = _run(ref_ctx, problem)
+    ref_ctx.close()
+    ctx = Context(0)
+    ctx.set_tuning(slots=slots, ql_zones=ql)
+    np.testing.assert_array_equal(_run(ctx, problem), ref)
+    ctx.close()
+
+
+def test_tuning_rejects_bad_values():
+    from oak_amd import OakError
+    from oak_amd._lib import Context
+    ctx = Context(0)
+    for kw in (dict(slots=0), dict(slots=5), dict(ql_zones=12), dict(graph=2)):
+        with pytest.raises(OakError):
+            ctx.set_tuning(**kw)
+    ctx.close()

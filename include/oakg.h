@@ -188,6 +188,30 @@ int oakg_local_analysis_csr(oakg_ctx* ctx, int64_t nzones, const int64_t* zone_p
                             double* xa, double* Sa, int64_t ldSa, double* amplitudes,
                             int32_t localise_obs, uint32_t flags);
 
+/* oakg_local_analysis_csr without any device allocation.  No counterpart in
+ * OAK: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as oakg_local_analysis_csr, with every array a
+ * device pointer (flags must hold OAKG_DEVICE_PTRS).  Every internal buffer
+ * (the observation-space copy HS, the per-zone matrices of the eigensolve,
+ * the error word) comes from one fixed device array of
+ * OAKG_NOALLOC_SCRATCH_BYTES per GPU, reserved when the library is loaded
+ * and held until the process ends; calls whose m, N and zone count need
+ * more than that (about m (NP + 2) 8 bytes + 4 bytes per zone + 64 zones of
+ * eigensolver workspace, 52 MiB at N = 128) return OAKG_EARG before any
+ * device work.  The zones run in chunks that fit, in order on the context's
+ * stream; no events, streams or phase timers are created (ms_total and the
+ * phase times read 0).  Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (other host threads, other contexts or
+ * streams) race and must be serialised by the caller. */
+#define OAKG_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int oakg_local_analysis_csr_noalloc(oakg_ctx* ctx, int64_t nzones, const int64_t* zone_ptr,
+                                    const int64_t* sel_ptr, const int32_t* sel_idx,
+                                    const double* sel_w, const oakg_obs* obs, int32_t N,
+                                    const double* xf, const double* Hxf, const double* Sf,
+                                    int64_t ldSf, const double* HSf, int64_t ldHSf, double* xa,
+                                    double* Sa, int64_t ldSa, double* amplitudes,
+                                    int32_t localise_obs, uint32_t flags);
 
 /* The selection alone (assimilation.F90:3683-3771) for every zone: writes
  * sel_ptr[nzones+1] and, if sel_idx/sel_w are not NULL, up to cap entries of

@@ -16,6 +16,7 @@ OAKG_OK, OAKG_EARG, OAKG_ENAN, OAKG_EHIP, OAKG_ENOMEM, OAKG_EUNSUP = range(6)
 OAKG_FP64, OAKG_MIXED = 0, 1
 OAKG_DEVICE_PTRS = 1
 OAKG_MAX_N = 128
+OAKG_NOALLOC_SCRATCH_BYTES = 256 << 20
 OAKG_EIG_JACOBI, OAKG_EIG_QL = 0, 1
 
 # Names every build of the library exports (checked by the CPU test suite).
@@ -25,7 +26,7 @@ EXPORTS = [
     "oakg_local_analysis_csr", "oakg_select_observations", "oakg_ensemble_analysis",
     "oakg_ensemble_analysis_f32", "oakg_local_nll", "oakg_report", "oakg_ensemble_report",
     "oakg_ensemble_report_f32", "oakg_coo_to_csr", "oakg_partition",
-    "oakg_obs_operator", "oakg_gather_rows",
+    "oakg_local_analysis_csr_noalloc", "oakg_obs_operator", "oakg_gather_rows",
     "oakg_scatter_rows", "oakg_set_tuning", "oakg_get_tuning",
 ]
 OAKG_TUNE_SLOTS, OAKG_TUNE_QL_ZONES, OAKG_TUNE_GRAPH = 1, 2, 3
@@ -118,6 +119,7 @@ def load():
     L.oakg_local_analysis_csr.argtypes = [
         _vp, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Obs), C.c_int32, _vp, _vp, _vp,
         C.c_int64, _vp, C.c_int64, _vp, _vp, C.c_int64, _vp, C.c_int32, C.c_uint32]
+    L.oakg_local_analysis_csr_noalloc.argtypes = L.oakg_local_analysis_csr.argtypes
     L.oakg_select_observations.argtypes = [
         _vp, C.POINTER(Zones), C.POINTER(Obs), C.c_int32, _vp, _vp, _vp, C.c_int64,
         C.POINTER(C.c_int64), C.c_uint32]

@@ -22,6 +22,25 @@
 #define OAKG_PIPE_SLOTS 4  // at most this many zone chunks in flight on as many streams
 #endif
 
+namespace oakg {
+// Bump allocator over a fixed device buffer (the no-allocation entry point's
+// scratch array): buffers of one call, 256-byte aligned, released together.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* get(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > cap || bytes > cap - at) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+  size_t left() const {
+    const size_t at = (used + 255) & ~(size_t)255;
+    return at < cap ? cap - at : 0;
+  }
+};
+}  // namespace oakg
+
 struct oakg_ctx {
   int device = 0;
   int precision = OAKG_FP64;
@@ -58,6 +77,9 @@ struct oakg_ctx {
   int64_t graph_replays = 0, graph_captures = 0;
   // events around each call's device work (its ms_total), created once
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  // set only during a call of oakg_local_analysis_csr_noalloc: every buffer
+  // of the call comes from this arena (the device's scratch array)
+  oakg::Arena* arena = nullptr;
 };
 
 namespace oakg {
@@ -161,8 +183,10 @@ void arg_fail(oakg_ctx* c, const std::string& msg) {
   throw Fail{OAKG_EARG};
 }
 
-// A device buffer of the call, from the context's grow-only workspace.
+// A device buffer of the call: from the context's grow-only workspace, or,
+// during a no-allocation call, from the scratch arena (nullptr when full).
 void* wsget(oakg_ctx* c, Workspace::Slot s, size_t bytes) {
+  if (c->arena) return c->arena->get(bytes == 0 ? 8 : bytes);
   return c->ws.get(s, bytes);
 }
 
@@ -290,12 +314,17 @@ void validate_obs(oakg_ctx* c, const oakg_obs* o, bool need_geo, int loctype) {
 void begin_call(oakg_ctx* c) {
   c->err.clear();
   check(c, hipSetDevice(c->device), "hipSetDevice");
-  if (!c->d_err) {
-    check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
-    check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
+  if (c->arena) {
+    // no-allocation call: the error word and the statistics come first in
+    // the scratch (NoallocCall swaps them in); no events are created
+  } else {
+    if (!c->d_err) {
+      check(c, hipMalloc(&c->d_err, 2 * sizeof(unsigned)), "hipMalloc");
+      check(c, hipMalloc(&c->d_stats, 8 * sizeof(int64_t)), "hipMalloc");
+    }
+    if (!c->ev_t0) check(c, hipEventCreate(&c->ev_t0), "event");
+    if (!c->ev_t1) check(c, hipEventCreate(&c->ev_t1), "event");
   }
-  if (!c->ev_t0) check(c, hipEventCreate(&c->ev_t0), "event");
-  if (!c->ev_t1) check(c, hipEventCreate(&c->ev_t1), "event");
   fill_u32(c->d_err, 2, 0u, c->stream);  // [0] error bits, [1] eigensolver fallbacks
   memset(&c->stats, 0, sizeof c->stats);
   c->counts_ready = true;
@@ -375,12 +404,23 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0);
   const size_t budget = c->ws_budget;
-  const int nslot = (!c->timing && nz >= 4096) ? c->tune_slots : 1;
+  const int nslot = (!c->timing && !c->arena && nz >= 4096) ? c->tune_slots : 1;
   int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
   zc = std::min<int64_t>(zc, 1 << 20);
   if (nslot > 1) zc = std::min<int64_t>(zc, (nz + nslot - 1) / nslot);  // a chunk per slot
   zc = std::min<int64_t>(zc, nz);
   zc = (zc + 63) & ~(int64_t)63;  // slots start on 64-zone groups of the rotation log
+  if (c->arena) {
+    // chunks of what is left of the scratch (7 buffers of up to 255 bytes of
+    // alignment), in whole 64-zone groups of the rotation log
+    const size_t left = c->arena->left();
+    const int64_t fit = left > 8 * 256 ? (int64_t)((left - 8 * 256) / per_zone) : 0;
+    zc = std::min<int64_t>(zc, fit & ~(int64_t)63);
+    if (zc < 64) {
+      c->err = "the no-allocation scratch is too small for one group of 64 zones at this N";
+      throw Fail{OAKG_EARG};
+    }
+  }
   const int64_t zs = zc * nslot;
   double* mat = (double*)wsget(c, Workspace::MAT, sizeof(double) * (size_t)NP * NP * zs);
   double* ymat = (double*)wsget(c, Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zs);
@@ -409,7 +449,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   // HIP graph: the launches below depend only on these values; the same key
   // replays the captured graph (on the context's capture stream, ordered
   // after the caller's stream and joined back into it)
-  const bool use_graph = c->graph && !c->timing;
+  const bool use_graph = c->graph && !c->timing && !c->arena;
   std::vector<char> key;
   if (use_graph) {
     struct Key {
@@ -561,7 +601,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   if (ldSf < n || ldSa < n || (m > 0 && ldHSf < m)) arg_fail(c, "leading dimension too small");
   begin_call(c);
   if (c->timing) c->timer.mark(PH_PREP);
-  check(c, hipEventRecord(c->ev_t0, c->stream), "event");
+  if (!c->arena) check(c, hipEventRecord(c->ev_t0, c->stream), "event");
 
   oakg_zones zz{};
   if (csr) {
@@ -656,10 +696,10 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   A.maxcorr = HUGE_VAL;
   run_zones(c, NP, A, nz, sweeps);
   if (c->timing) c->timer.mark(PH_END);
-  check(c, hipEventRecord(c->ev_t1, c->stream), "event");
+  if (!c->arena) check(c, hipEventRecord(c->ev_t1, c->stream), "event");
   finish_call(c, nz, sel.ptr, sweeps);
   float ms = 0;
-  (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
+  if (!c->arena) (void)hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1);
   collect_stats(c, nz, sel.ptr, ms);
   (void)total;
   if (!dev) {
@@ -1315,6 +1355,75 @@ int permute_rows_impl(oakg_ctx* c, bool gather, int64_t n, int32_t N, const int6
   return OAKG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point: one fixed array per device,
+// reserved when the code object is loaded and held until the process ends.
+// Its device address is looked up once per device (with that device
+// current); host code uses only that address, never the array's name.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = OAKG_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch address on the context's device (set up on the first call
+// there).  The device must be current.
+void* scratch_on_device(oakg_ctx* c) {
+  if (c->device < 0 || c->device >= kMaxDevices) arg_fail(c, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void* ptr = nullptr;
+    check(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) check(c, hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[c->device] = ptr;
+  }
+  return g_scratch_ptr[c->device];
+}
+
+// Bytes of scratch a no-allocation call needs besides its zone chunks.
+size_t noalloc_fixed_bytes(int64_t m, int NP, int64_t nz) {
+  const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t mm = (size_t)std::max<int64_t>(m, 1);
+  return al(256) + al(sizeof(double) * mm * NP) + 2 * al(sizeof(double) * mm) +
+         al(sizeof(int32_t) * (size_t)std::max<int64_t>(nz, 1));
+}
+
+// Bytes of the zone-chunk workspace per zone (as run_zones sizes it).
+size_t noalloc_zone_bytes(int NP, bool ql) {
+  const int64_t cap = (((int64_t)2 * NP * NP + 8 * NP) + 3) & ~(int64_t)3;
+  return sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) + (ql ? (size_t)cap * 16 + 4 : 0);
+}
+
+// For the duration of a no-allocation call: the arena over the scratch, the
+// error word and statistics at its head, no phase timing.
+struct NoallocCall {
+  oakg_ctx* c;
+  Arena arena;
+  unsigned* save_err;
+  int64_t* save_stats;
+  bool save_timing;
+  NoallocCall(oakg_ctx* cc, void* base) : c(cc) {
+    arena.base = (char*)base;
+    arena.cap = kScratchBytes;
+    save_err = c->d_err;
+    save_stats = c->d_stats;
+    save_timing = c->timing;
+    char* head = (char*)arena.get(256);
+    c->d_err = (unsigned*)head;               // [0] error bits, [1] eigensolver fallbacks
+    c->d_stats = (int64_t*)(head + 64);       // 8 statistics
+    c->timing = false;
+    c->arena = &arena;
+  }
+  ~NoallocCall() {
+    c->arena = nullptr;
+    c->d_err = save_err;
+    c->d_stats = save_stats;
+    c->timing = save_timing;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1476,6 +1585,35 @@ int oakg_local_analysis_csr(oakg_ctx* c, int64_t nzones, const int64_t* zone_ptr
                             int32_t localise_obs, uint32_t flags) {
   return guarded(c, [&] {
     if (!obs || !sel_ptr || !zone_ptr) arg_fail(c, "obs, zone_ptr and sel_ptr are required");
+    return local_analysis_impl(c, nullptr, nzones, zone_ptr, sel_ptr, sel_idx, sel_w, obs, N, xf,
+                               Hxf, Sf, ldSf, HSf, ldHSf, xa, Sa, ldSa, amplitudes, localise_obs,
+                               flags);
+  });
+}
+
+int oakg_local_analysis_csr_noalloc(oakg_ctx* c, int64_t nzones, const int64_t* zone_ptr,
+                                    const int64_t* sel_ptr, const int32_t* sel_idx,
+                                    const double* sel_w, const oakg_obs* obs, int32_t N,
+                                    const double* xf, const double* Hxf, const double* Sf,
+                                    int64_t ldSf, const double* HSf, int64_t ldHSf, double* xa,
+                                    double* Sa, int64_t ldSa, double* amplitudes,
+                                    int32_t localise_obs, uint32_t flags) {
+  return guarded(c, [&] {
+    if (!(flags & OAKG_DEVICE_PTRS))
+      arg_fail(c, "oakg_local_analysis_csr_noalloc takes device pointers only (OAKG_DEVICE_PTRS)");
+    if (!obs || !sel_ptr || !zone_ptr) arg_fail(c, "obs, zone_ptr and sel_ptr are required");
+    if (N < 1 || N > OAKG_MAX_N) arg_fail(c, "N out of range");
+    const int NP = padded_n(N);
+    if (nzones < 0 || obs->m < 0) arg_fail(c, "negative nzones or m");
+    const size_t need = noalloc_fixed_bytes(obs->m, NP, nzones) +
+                        64 * noalloc_zone_bytes(NP, c->eig_method == 1) + 8 * 256;
+    if (need > kScratchBytes)
+      arg_fail(c, "oakg_local_analysis_csr_noalloc: m = " + std::to_string(obs->m) + ", N = " +
+                      std::to_string(N) + " and " + std::to_string(nzones) + " zones need " +
+                      std::to_string(need >> 20) + " MiB of scratch, above its " +
+                      std::to_string(kScratchBytes >> 20) + " MiB");
+    check(c, hipSetDevice(c->device), "hipSetDevice");
+    NoallocCall guard(c, scratch_on_device(c));
     return local_analysis_impl(c, nullptr, nzones, zone_ptr, sel_ptr, sel_idx, sel_w, obs, N, xf,
                                Hxf, Sf, ldSf, HSf, ldHSf, xa, Sa, ldSa, amplitudes, localise_obs,
                                flags);

@@ -157,12 +157,22 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
  *                         while its kernel arguments and buffers stay the
  *                         same (a caller that repeats an analysis on the same
  *                         device buffers); 0: launched eagerly (default)
+ *   OAKG_TUNE_ORDER       1: zones run in order of decreasing observation
+ *                         count (quarter-octave buckets; every zone chunk
+ *                         gets the same mix), the heaviest first; 0: in
+ *                         zone order (default)
+ *   OAKG_TUNE_TRIDIAG_SPLIT  1: at N > 64 the Householder reduction runs as
+ *                         two kernels (the first half of the steps, then the
+ *                         rest with fewer registers and more zones per CU);
+ *                         0: one kernel (default)
  * oakg_get_tuning also reads OAKG_TUNE_GRAPH_REPLAYS / _CAPTURES (counts). */
 #define OAKG_TUNE_SLOTS 1
 #define OAKG_TUNE_QL_ZONES 2
 #define OAKG_TUNE_GRAPH 3
 #define OAKG_TUNE_GRAPH_REPLAYS 4
 #define OAKG_TUNE_GRAPH_CAPTURES 5
+#define OAKG_TUNE_ORDER 6
+#define OAKG_TUNE_TRIDIAG_SPLIT 7
 int oakg_set_tuning(oakg_ctx* ctx, int key, int value);
 int oakg_get_tuning(oakg_ctx* ctx, int key, int64_t* value);
 /* Per-zone local observation counts p_z of the last call (host array). */

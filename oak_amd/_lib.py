@@ -30,7 +30,8 @@ EXPORTS = [
     "oakg_scatter_rows", "oakg_set_tuning", "oakg_get_tuning",
 ]
 OAKG_TUNE_SLOTS, OAKG_TUNE_QL_ZONES, OAKG_TUNE_GRAPH = 1, 2, 3
-OAKG_TUNE_GRAPH_REPLAYS, OAKG_TUNE_GRAPH_CAPTURES = 4, 5
+OAKG_TUNE_GRAPH_REPLAYS, OAKG_TUNE_GRAPH_CAPTURES, OAKG_TUNE_ORDER = 4, 5, 6
+OAKG_TUNE_TRIDIAG_SPLIT = 7
 
 
 class OakError(RuntimeError):
@@ -193,12 +194,14 @@ class Context:
         """OAKG_EIG_QL (default) or OAKG_EIG_JACOBI."""
         self.check(load().oakg_set_eigensolver(self._h, int(method)))
 
-    def set_tuning(self, slots=None, ql_zones=None, graph=None):
+    def set_tuning(self, slots=None, ql_zones=None, graph=None, order=None, tri_split=None):
         """Zone chunks in flight, zones per QL wave, HIP-graph replay of the
-        zone loop (oakg_set_tuning)."""
+        zone loop, zones ordered by observation count, the N > 64 Householder
+        reduction as two kernels (oakg_set_tuning)."""
         L = load()
         for key, v in ((OAKG_TUNE_SLOTS, slots), (OAKG_TUNE_QL_ZONES, ql_zones),
-                       (OAKG_TUNE_GRAPH, graph)):
+                       (OAKG_TUNE_GRAPH, graph), (OAKG_TUNE_ORDER, order),
+                       (OAKG_TUNE_TRIDIAG_SPLIT, tri_split)):
             if v is not None:
                 self.check(L.oakg_set_tuning(self._h, key, int(v)))
 
@@ -206,7 +209,9 @@ class Context:
         L = load()
         out = {}
         for name, key in (("slots", OAKG_TUNE_SLOTS), ("ql_zones", OAKG_TUNE_QL_ZONES),
-                          ("graph", OAKG_TUNE_GRAPH), ("graph_replays", OAKG_TUNE_GRAPH_REPLAYS),
+                          ("graph", OAKG_TUNE_GRAPH), ("order", OAKG_TUNE_ORDER),
+                          ("tri_split", OAKG_TUNE_TRIDIAG_SPLIT),
+                          ("graph_replays", OAKG_TUNE_GRAPH_REPLAYS),
                           ("graph_captures", OAKG_TUNE_GRAPH_CAPTURES)):
             v = C.c_int64(0)
             self.check(L.oakg_get_tuning(self._h, key, C.byref(v)))

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_obs_anom(int64_t m, int N, const double
 // values are exact in FP64, so the FP64 accumulation adds no FP32 error).
 // ---------------------------------------------------------------------------
 template <int NP, typename TS>
-__global__ __launch_bounds__(256) void k_gram(int64_t z0, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256) void k_gram(int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                               const int32_t* __restrict__ sel_idx,
                                               const double* __restrict__ sel_w, int64_t m,
                                               const double* __restrict__ rho,
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_gram(int64_t z0, const int64_t* __restr
   constexpr int NT = NB * (NB + 1) / 2;
   __shared__ double red[4][256];
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int64_t p0 = sel_ptr[z];
   const int64_t p = sel_ptr[z + 1] - p0;
   double* M = mat + (size_t)zl * NP * NP;
@@ -300,7 +300,7 @@ __device__ __forceinline__ double colsum_full(EigLds<NP>& L, int lane, int gb, i
 }
 
 template <int NP>
-__global__ __launch_bounds__(256) void k_eig(int64_t z0, int nzc, int N,
+__global__ __launch_bounds__(256) void k_eig(int64_t z0, const int32_t* __restrict__ zorder, int nzc, int N,
                                              const int64_t* __restrict__ sel_ptr,
                                              const double* __restrict__ mat,
                                              double* __restrict__ ymat, double* __restrict__ vec,
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_eig(int64_t z0, int nzc, int N,
   const int zl = (blockIdx.x * 4 + wave) * GPW + grp;
   // with `only`, zones whose flag has no bit in 3 are left untouched
   const bool valid = zl < nzc && (!only || (only[zl] & 3));
-  const int64_t z = z0 + (valid ? zl : 0);
+  const int64_t z = zone_at(zorder, z0 + (valid ? zl : 0));
   const bool active = valid && (sel_ptr[z + 1] > sel_ptr[z]);
   const double* M = mat + (size_t)(valid ? zl : 0) * NP * NP;
   double x[NP];
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_eig(int64_t z0, int nzc, int N,
 // Zones without observations get T = I exactly (Sa = Sf, rrsqrt.F90:326,371).
 // ---------------------------------------------------------------------------
 template <int NP>
-__global__ __launch_bounds__(256) void k_tmat(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256) void k_tmat(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
                                               const double* __restrict__ ymat,
                                               double* __restrict__ mat,
                                               const double* __restrict__ vec) {
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_tmat(int64_t z0, int N, const int64_t* 
   __shared__ double T0[NP * LD];
   __shared__ double U[4][NP];
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   double* M = mat + (size_t)zl * NP * NP;
   const bool active = sel_ptr[z + 1] > sel_ptr[z];
   if (!active) {
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(256) void k_tmat(int64_t z0, int N, const int64_t* 
 //             Ea = xa + sqrt(N-1) * (inflation * Sf T)  (assimilation.F90:3301-3320)
 // ---------------------------------------------------------------------------
 template <int NP, int MODE>
-__global__ __launch_bounds__(256) void k_update(int64_t z0, int N, const int64_t* __restrict__ zone_ptr,
+__global__ __launch_bounds__(256) void k_update(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ zone_ptr,
                                                 const double* __restrict__ mat,
                                                 const double* __restrict__ vec,
                                                 const double* __restrict__ X0,  // xf or xbar
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(256) void k_update(int64_t z0, int N, const int64_t
   constexpr int LDW = NP + 17;
   __shared__ double Ws[NP * LDW];
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int64_t i0 = zone_ptr[z];
   const int64_t sz = zone_ptr[z + 1] - i0;
   const double* T = mat + (size_t)zl * NP * NP;
@@ -662,20 +662,20 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   constexpr int GPW = 64 / NP;
   if (tm) tm->mark(PH_GRAM);
   if (A.prec == 1)
-    hipLaunchKernelGGL((k_gram<NP, float>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+    hipLaunchKernelGGL((k_gram<NP, float>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
   else
-    hipLaunchKernelGGL((k_gram<NP, double>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+    hipLaunchKernelGGL((k_gram<NP, double>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
   if (tm) tm->mark(PH_EIG);
   if (A.eig_method == 1) {
     launch_tql_pipeline(NP, A, st);
     // zones the QL could not finish fall back to Jacobi
-    hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.nzc,
+    hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.zorder, A.nzc,
                        A.N, A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
                        A.max_sweeps, (const int32_t*)A.tql_status);
   } else {
-    hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.nzc,
+    hipLaunchKernelGGL(k_eig<NP>, dim3(nblk(A.nzc, 4 * GPW)), dim3(256), 0, st, A.z0, A.zorder, A.nzc,
                        A.N, A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
                        A.max_sweeps, (const int32_t*)nullptr);
   }
@@ -686,17 +686,17 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
     return;
   }
   if (tm) tm->mark(PH_TMAT);
-  hipLaunchKernelGGL(k_tmat<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.ymat,
+  hipLaunchKernelGGL(k_tmat<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr, A.ymat,
                      A.mat, A.vec);
   if (tm) tm->mark(PH_UPDATE);
   if (A.prec == 1)
     launch_update_f32(NP, A, st);
   else if (A.mode == 0)
-    hipLaunchKernelGGL((k_update<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr,
+    hipLaunchKernelGGL((k_update<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.zone_ptr,
                        A.mat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl,
                        A.maxcorr, A.maxcorr_v);
   else
-    hipLaunchKernelGGL((k_update<NP, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr,
+    hipLaunchKernelGGL((k_update<NP, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.zone_ptr,
                        A.mat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling, A.infl,
                        A.maxcorr, A.maxcorr_v);
   if (tm) tm->mark(PH_END);

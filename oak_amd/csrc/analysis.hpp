@@ -72,8 +72,16 @@ struct PrepEnsArgs {
   float* HS32;
 };
 
+// The zone at position pos of the zone loop: zones run in the order of
+// zorder when the loop has one (zones grouped by observation count), else in
+// their own order.
+__device__ __forceinline__ int64_t zone_at(const int32_t* __restrict__ zorder, int64_t pos) {
+  return zorder ? (int64_t)zorder[pos] : pos;
+}
+
 struct ChunkArgs {
-  int64_t z0;
+  int64_t z0;               // first position of the chunk in the zone loop
+  const int32_t* zorder;    // position -> zone, or NULL (position = zone)
   int nzc, N, mode;  // mode 0 = locAnalysis, 1 = ensemble
   int64_t m;
   const int64_t* zone_ptr;
@@ -86,6 +94,7 @@ struct ChunkArgs {
   // eigensolver: 0 = one-sided Jacobi, 1 = Householder tridiagonalisation + implicit QL
   int eig_method;
   int ql_zw;                // zones per wave of the QL kernel (8, 16 or 32)
+  int tri_split;            // NP = 128: the Householder reduction as two launches
   double *dvec, *evec;      // tridiagonal / eigenvalues, NP x nzc
   void* rlog;               // QL rotation log, rlog_cap records of 16 B per zone
   int64_t rlog_cap;

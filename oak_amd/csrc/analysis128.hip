@@ -118,7 +118,7 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
 }
 
 template <typename TS>
-__global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                                  const int32_t* __restrict__ sel_idx,
                                                  const double* __restrict__ sel_w, int64_t m,
                                                  const double* __restrict__ rho,
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int64_t* __re
                                                  double* __restrict__ mat, double* __restrict__ vec,
                                                  unsigned* __restrict__ err) {
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int64_t p0 = sel_ptr[z];
   const int64_t p = sel_ptr[z + 1] - p0;
   double* M = mat + (size_t)zl * P128 * P128;
@@ -192,7 +192,7 @@ __device__ __forceinline__ void jacobi_cs(double gam, double alpha, double beta,
   }
 }
 
-__global__ __launch_bounds__(256) void k_eig128(int64_t z0, int nzc, int N,
+__global__ __launch_bounds__(256) void k_eig128(int64_t z0, const int32_t* __restrict__ zorder, int nzc, int N,
                                                 const int64_t* __restrict__ sel_ptr,
                                                 const double* __restrict__ mat,
                                                 double* __restrict__ ymat, double* __restrict__ vec,
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_eig128(int64_t z0, int nzc, int N,
                                                 const int32_t* __restrict__ only) {
   __shared__ Eig128Lds L;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   if (only && !(only[zl] & 3)) return;
   double* V = vec + (size_t)zl * 4 * P128;
@@ -478,7 +478,7 @@ struct XformLds {
 };
 
 template <int MODE, typename TE>
-__global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(512) void k_xform128(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
                                                   const int64_t* __restrict__ zone_ptr,
                                                   const double* __restrict__ ymat,
                                                   const double* __restrict__ vec,
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, int N, const int64
   static_assert(!F32, "FP32 state runs in k_xform128f");
   __shared__ XformLds L;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, kk = lane >> 4;
   const int64_t i0 = zone_ptr[z];
@@ -735,7 +735,7 @@ struct XformLdsF {
   double a[P128 + 2];  // a / scaling, then v.w~
 };
 
-__global__ __launch_bounds__(256) void k_xform128f(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256) void k_xform128f(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
                                                    const int64_t* __restrict__ zone_ptr,
                                                    const double* __restrict__ ymat,
                                                    const double* __restrict__ vec,
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, int N, const int6
 #pragma clang fp contract(off)
   __shared__ XformLdsF L;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, kk = lane >> 4;
   const int64_t i0 = zone_ptr[z];
@@ -946,19 +946,19 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, int N, const int6
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_GRAM);
   if (A.prec == 1)
-    hipLaunchKernelGGL(k_gram128<float>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+    hipLaunchKernelGGL(k_gram128<float>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
   else
-    hipLaunchKernelGGL(k_gram128<double>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr,
+    hipLaunchKernelGGL(k_gram128<double>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
   if (tm) tm->mark(PH_EIG);
   if (A.eig_method == 1) {
     launch_tql_pipeline(128, A, st);
-    hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.nzc, A.N, A.sel_ptr,
+    hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.nzc, A.N, A.sel_ptr,
                        A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps,
                        (const int32_t*)A.tql_status);
   } else {
-    hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.nzc, A.N, A.sel_ptr,
+    hipLaunchKernelGGL(k_eig128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.nzc, A.N, A.sel_ptr,
                        A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps,
                        (const int32_t*)nullptr);
   }
@@ -971,15 +971,15 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   // transform and update in one kernel (the timer books it as the update)
   if (tm) tm->mark(PH_UPDATE);
   if (A.prec == 1)
-    hipLaunchKernelGGL(k_xform128f, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr,
+    hipLaunchKernelGGL(k_xform128f, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S32, A.ldS, A.Out32, A.ldO, A.scaling,
                        A.infl, A.maxcorr, A.maxcorr_v);
   else if (A.mode == 0)
-    hipLaunchKernelGGL((k_xform128<0, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
+    hipLaunchKernelGGL((k_xform128<0, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
                        A.infl, A.maxcorr, A.maxcorr_v);
   else
-    hipLaunchKernelGGL((k_xform128<1, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.N, A.sel_ptr,
+    hipLaunchKernelGGL((k_xform128<1, double>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
                        A.infl, A.maxcorr, A.maxcorr_v);
   if (tm) tm->mark(PH_END);

@@ -67,6 +67,8 @@ struct oakg_ctx {
   // tuning knobs (oakg_set_tuning): zone chunks in flight, zones per QL wave
   int tune_slots = 3;
   int tune_ql_zw = 16;
+  int tune_order = 0;  // 1: zones ordered by observation count (k_zone_order)
+  int tune_tri_split = 0;  // 1: N = 128 Householder reduction as two launches
   // the zone loop captured in a HIP graph and replayed while its launches
   // (kernel arguments, buffers, streams) stay the same
   bool graph = false;
@@ -152,6 +154,46 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// Order of the zone loop by observation count p: buckets of a quarter octave
+// of p, the largest counts first, empty zones last; the ranks are dealt
+// round-robin to the K chunks of the loop, chunk c holding positions
+// c q + min(c, r) .. (+ q + (c < r)) with nz = K q + r, so that every chunk
+// gets the same mix and the heaviest zones of each chunk start first.  One
+// workgroup; the order inside a bucket is that of the LDS atomics, which
+// changes the schedule and never the results (each zone is computed alone).
+constexpr int kOrdBuckets = 64;
+__device__ __forceinline__ int ord_bucket(int64_t p) {
+  if (p <= 0) return kOrdBuckets - 1;
+  const int e = 63 - __clzll((unsigned long long)p);  // floor(log2 p)
+  const int f = e >= 2 ? (int)((p >> (e - 2)) & 3) : (int)((p << (2 - e)) & 3);
+  return max(0, kOrdBuckets - 2 - (4 * e + f));  // p = 1 -> 62, p = 8000 -> 11
+}
+__global__ __launch_bounds__(1024) void k_zone_order(int64_t nz, int64_t K,
+                                                     const int64_t* __restrict__ sel_ptr,
+                                                     int32_t* __restrict__ zorder) {
+  __shared__ unsigned long long cnt[kOrdBuckets];
+  if (threadIdx.x < kOrdBuckets) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t z = threadIdx.x; z < nz; z += blockDim.x)
+    atomicAdd(&cnt[ord_bucket(sel_ptr[z + 1] - sel_ptr[z])], 1ull);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long acc = 0;
+    for (int b = 0; b < kOrdBuckets; ++b) {
+      const unsigned long long k = cnt[b];
+      cnt[b] = acc;  // first rank of the bucket
+      acc += k;
+    }
+  }
+  __syncthreads();
+  const int64_t q = nz / K, r = nz % K;
+  for (int64_t z = threadIdx.x; z < nz; z += blockDim.x) {
+    const int64_t i = (int64_t)atomicAdd(&cnt[ord_bucket(sel_ptr[z + 1] - sel_ptr[z])], 1ull);
+    const int64_t ch = i % K, j = i / K;
+    zorder[ch * q + min(ch, r) + j] = (int32_t)z;
+  }
+}
+
 void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
@@ -436,6 +478,14 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     tst = (int32_t*)wsget(c, Workspace::TQL_STATUS, sizeof(int32_t) * zs);
     if (!dvec || !evec || !rlog || !tst) check(c, hipErrorOutOfMemory, "workspace");
   }
+  // zone order (not in a no-allocation call: its scratch is sized without it)
+  const bool order = c->tune_order == 1 && !c->arena && nz > 1 && nz <= INT32_MAX;
+  const int64_t nchunk = nz > 0 ? (nz + zc - 1) / zc : 0;
+  int32_t* zorder = nullptr;
+  if (order) {
+    zorder = (int32_t*)wsget(c, Workspace::ZORDER, sizeof(int32_t) * (size_t)nz);
+    if (!zorder) check(c, hipErrorOutOfMemory, "workspace");
+  }
   // streams and events of the pipeline, created before any capture
   if (nslot > 1) {
     if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
@@ -455,8 +505,8 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     struct Key {
       ChunkArgs base;
       int64_t nz, zc, cap;
-      int NP, nslot, ql, ql_zw;
-      const void* buf[8];
+      int NP, nslot, ql, ql_zw, order, tri_split;
+      const void* buf[9];
       hipStream_t stream;
     } kk;
     memset(&kk, 0, sizeof kk);
@@ -468,7 +518,9 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     kk.nslot = nslot;
     kk.ql = ql;
     kk.ql_zw = c->tune_ql_zw;
-    const void* bufs[8] = {mat, ymat, vec, dvec, evec, rlog, tst, sweeps_all};
+    kk.order = order;
+    kk.tri_split = c->tune_tri_split;
+    const void* bufs[9] = {mat, ymat, vec, dvec, evec, rlog, tst, sweeps_all, zorder};
     memcpy(kk.buf, bufs, sizeof bufs);
     kk.stream = c->stream;
     key.assign((const char*)&kk, (const char*)&kk + sizeof kk);
@@ -500,6 +552,10 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     }
   } guard{s0, use_graph};
   if (ql) fill_i32(sweeps_all, nz, 0, s0);
+  if (order) {
+    hipLaunchKernelGGL(k_zone_order, dim3(1), dim3(1024), 0, s0, nz, nchunk, base.sel_ptr, zorder);
+    check(c, hipGetLastError(), "zone order");
+  }
   hipStream_t S[OAKG_PIPE_SLOTS];
   for (int i = 0; i < OAKG_PIPE_SLOTS; ++i) S[i] = s0;
   if (nslot > 1) {
@@ -509,13 +565,17 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
       check(c, hipStreamWaitEvent(S[i], c->ev_fork, 0), "stream wait");
     }
   }
-  int64_t k = 0;
-  for (int64_t z0 = 0; z0 < nz; z0 += zc, ++k) {
+  // chunk k: positions z0 .. z0 + nzc - 1 of the loop (with an order, the
+  // layout k_zone_order deals: nz = nchunk q + r, at most zc zones each)
+  const int64_t oq = nchunk ? nz / nchunk : 0, orr = nchunk ? nz % nchunk : 0;
+  for (int64_t k = 0; k < nchunk; ++k) {
     const int sl = (int)(k % nslot);
     const size_t zo = (size_t)sl * zc;  // first zone of the slot's workspace
+    const int64_t z0 = order ? k * oq + std::min(k, orr) : k * zc;
     ChunkArgs A = base;
     A.z0 = z0;
-    A.nzc = (int)std::min<int64_t>(zc, nz - z0);
+    A.zorder = zorder;
+    A.nzc = (int)(order ? oq + (k < orr ? 1 : 0) : std::min<int64_t>(zc, nz - z0));
     A.mat = mat + zo * NP * NP;
     A.ymat = ymat + zo * NP * NP;
     A.vec = vec + zo * 4 * NP;
@@ -527,6 +587,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     A.rlog_cap = cap;
     A.tql_status = ql ? tst + zo : nullptr;
     A.ql_zw = c->tune_ql_zw;
+    A.tri_split = c->tune_tri_split;
     launch_chunk(NP, A, S[sl], c->timing ? &c->timer : nullptr);
     check(c, hipGetLastError(), "chunk launch");
   }
@@ -1520,6 +1581,14 @@ int oakg_set_tuning(oakg_ctx* c, int key, int value) {
       if (value != 8 && value != 16 && value != 32) break;
       c->tune_ql_zw = value;
       return OAKG_OK;
+    case OAKG_TUNE_ORDER:
+      if (value != 0 && value != 1) break;
+      c->tune_order = value;
+      return OAKG_OK;
+    case OAKG_TUNE_TRIDIAG_SPLIT:
+      if (value != 0 && value != 1) break;
+      c->tune_tri_split = value;
+      return OAKG_OK;
     case OAKG_TUNE_GRAPH:
       if (value != 0 && value != 1) break;
       c->graph = value != 0;
@@ -1542,6 +1611,8 @@ int oakg_get_tuning(oakg_ctx* c, int key, int64_t* value) {
     case OAKG_TUNE_SLOTS: *value = c->tune_slots; return OAKG_OK;
     case OAKG_TUNE_QL_ZONES: *value = c->tune_ql_zw; return OAKG_OK;
     case OAKG_TUNE_GRAPH: *value = c->graph ? 1 : 0; return OAKG_OK;
+    case OAKG_TUNE_ORDER: *value = c->tune_order; return OAKG_OK;
+    case OAKG_TUNE_TRIDIAG_SPLIT: *value = c->tune_tri_split; return OAKG_OK;
     case OAKG_TUNE_GRAPH_REPLAYS: *value = c->graph_replays; return OAKG_OK;
     case OAKG_TUNE_GRAPH_CAPTURES: *value = c->graph_captures; return OAKG_OK;
     default: break;

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_obs_anom_f32(int64_t m, int N, const do
 // the transposed product (A = [T|a]^T from LDS, B = Sf^T) so that each
 // 16-row strip is stored as contiguous runs of the column-major Ea.
 template <int NP>
-__global__ __launch_bounds__(256) void k_update_f32(int64_t z0, int N,
+__global__ __launch_bounds__(256) void k_update_f32(int64_t z0, const int32_t* __restrict__ zorder, int N,
                                                     const int64_t* __restrict__ zone_ptr,
                                                     const double* __restrict__ mat,
                                                     const double* __restrict__ vec,
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_update_f32(int64_t z0, int N,
   __shared__ float Ws[NP * LDW];
   __shared__ double as[NP];
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int64_t i0 = zone_ptr[z];
   const int64_t sz = zone_ptr[z + 1] - i0;
   const double* T = mat + (size_t)zl * NP * NP;
@@ -202,7 +202,7 @@ void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st) {
 
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st) {
 #define OAK_UPD32(P)                                                                             \
-  hipLaunchKernelGGL(k_update_f32<P>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.zone_ptr, A.mat, \
+  hipLaunchKernelGGL(k_update_f32<P>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.zone_ptr, A.mat, \
                      A.vec, A.X0, A.S32, A.ldS, A.Out32, A.ldO, A.scaling, A.infl, A.maxcorr, \
                      A.maxcorr_v)
   switch (NP) {

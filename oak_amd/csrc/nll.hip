@@ -34,7 +34,7 @@ __device__ __forceinline__ double nll_wave_sum(double s) {
 // One workgroup (4 waves) per zone; wave w takes the observations w, w+4, ...
 // and lane j holds entries j, j+64, ... of the N-vectors.
 template <int NP>
-__global__ __launch_bounds__(256) void k_nll_adjoint(int64_t z0, int N,
+__global__ __launch_bounds__(256) void k_nll_adjoint(int64_t z0, const int32_t* __restrict__ zorder, int N,
                                                      const int64_t* __restrict__ sel_ptr,
                                                      const int32_t* __restrict__ sel_idx,
                                                      const double* __restrict__ sel_w,
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_nll_adjoint(int64_t z0, int N,
   __shared__ double sw[4][NP];
   __shared__ double red[4][3];
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int64_t p0 = sel_ptr[z];
   const int64_t p = sel_ptr[z + 1] - p0;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_nll_adjoint(int64_t z0, int N,
 
 void launch_nll_adjoint(int NP, const ChunkArgs& A, hipStream_t st) {
 #define OAK_NLL(P)                                                                                \
-  hipLaunchKernelGGL(k_nll_adjoint<P>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr,       \
+  hipLaunchKernelGGL(k_nll_adjoint<P>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,       \
                      A.sel_idx, A.sel_w, A.rho, A.innov, A.HS, A.ymat, A.vec, A.corr, A.infl,     \
                      A.nll_J, A.nll_gL, A.nll_gA, A.nll_gth)
   switch (NP) {

@@ -75,7 +75,9 @@ __device__ __forceinline__ double rcp_acc(double x) {
 // fall four to a bank.
 __host__ __device__ constexpr int tri_rx(int r) { return r + (r >> 3); }
 
-template <int NP>
+// XF: side of the largest trailing block moved between stages (NP/2 from
+// the first stage, NP/4 when a launch starts at the second).
+template <int NP, int XF = NP / 2>
 struct TriLds {
   static constexpr int PR = NP + NP / 8;  // padded length of a row vector
   double colw[4][PR]; // each wave's copy of v (reduction)
@@ -84,7 +86,7 @@ struct TriLds {
   double pw[2][PR];   // p (then y during the Q accumulation)
   union {
     double part[16][PR + 1];             // per-tile partial sums
-    double xfer[NP / 2][NP / 2 + 1];     // the trailing half between the stages
+    double xfer[XF][XF + 1];             // the trailing block between the stages
   };
   double red[2][4];   // per-wave shares of v^T A v (double-buffered)
   double tau[NP];
@@ -105,8 +107,17 @@ __device__ __forceinline__ void ring_tile(int t, int& tr, int& tc) {
   else { tr = mn + (j - s); tc = mn; }
 }
 
-template <int NP>
-__global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __restrict__ sel_ptr,
+// PART selects the steps a launch runs: 0 all of them (and, for NP < 128,
+// the accumulation of Q); for NP = 128 the reduction can run as two
+// launches, PART 1 the first stage (the 8 x 8 tiles that need most of the
+// registers) ending with the trailing half-block [H2, NP)^2 stored in the
+// rows H2 .. NP-1 of qmat (not yet holding reflectors: row r, column c - H2),
+// then PART 2 the later stages from that block.  PART 2 needs a fraction of
+// the registers, so more zones share a CU through its latency-bound steps.
+// Both ways run the same operations on the same values: bit-identical T
+// and reflectors.
+template <int NP, int PART>
+__global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                                  const double* __restrict__ mat,
                                                  double* __restrict__ qmat,
                                                  double* __restrict__ dvec, double* __restrict__ evec,
@@ -118,9 +129,9 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   constexpr int T2 = T1 >= 2 ? T1 / 2 : T1;
   constexpr int T3 = T2 >= 2 ? T2 / 2 : T2;
   constexpr int H2 = NP - 16 * T2, H3 = NP - 16 * T3;
-  __shared__ TriLds<NP> S;
+  __shared__ TriLds<NP, PART == 2 ? NP / 4 : NP / 2> S;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   int tr, tc;
@@ -369,11 +380,24 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
   using K2 = std::integral_constant<int, T2>;
   using K3 = std::integral_constant<int, T3>;
 
+  static_assert(PART == 0 || (NP == 128 && T2 < T1), "the two-launch reduction is for NP = 128");
   // ---- reduction ----
   double a3[T3][T3];
   {
     double a2[T2][T2];
-    {
+    if constexpr (PART == 2) {
+      // the trailing block and its first column from the rows of qmat
+#pragma unroll
+      for (int ii = 0; ii < T2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < T2; ++jj) a2[ii][jj] = V[(size_t)(r2 + ii) * NP + (c2 + jj - H2)];
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int r = lane + 64 * h;
+        x[h] = (r >= H2 && r < NP) ? V[(size_t)r * NP] : 0.0;
+      }
+      __syncthreads();  // every read of the block ends before row H2 gets its reflector
+    } else {
       double a[T1][T1];
 #pragma unroll
       for (int ii = 0; ii < T1; ++ii)
@@ -386,6 +410,17 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int64_t* __re
     }
       if constexpr (T2 < T1) {
         reduce_steps(a, K1{}, r1, c1, 0, 0, H2);
+        if constexpr (PART == 1) {
+          // the trailing block to rows H2 .. NP-1 of qmat (the tiles, updated
+          // by step H2-1, hold it; the column x carries is its column 0)
+          if (r1 >= H2 && c1 >= H2) {
+#pragma unroll
+            for (int ii = 0; ii < T1; ++ii)
+#pragma unroll
+              for (int jj = 0; jj < T1; ++jj) V[(size_t)(r1 + ii) * NP + (c1 + jj - H2)] = a[ii][jj];
+          }
+          return;
+        }
         move_block(a, K1{}, r1, c1, a2, K2{}, r2, c2, H2);
       } else {
         copy_tile(a, a2, K1{});
@@ -469,12 +504,12 @@ struct FormQLds {
   double tau[2][16];
 };
 
-__global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                                   double* __restrict__ qmat) {
   constexpr int NP = 128, NB = 8, LV = FormQLds::LV;
   __shared__ FormQLds S;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int lr = lane >> 4, lc = lane & 15;
@@ -610,7 +645,7 @@ __device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) 
 // negligibility test of e_j is made as soon as e_j, d_j and d_{j+1} are final
 // in the sweep, so a full rescan is only needed after a 1 x 1 deflation.
 template <int NP, int ZW>
-__global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                             double* __restrict__ dvec, double* __restrict__ evec,
                                             double2* __restrict__ rlog, int64_t cap,
                                             int32_t* __restrict__ status,
@@ -623,7 +658,8 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
   const int lane = threadIdx.x;
   if (lane >= ZW) return;
   const int zl = blockIdx.x * ZW + lane;
-  const bool valid = zl < nzc && sel_ptr[z0 + zl + 1] > sel_ptr[z0 + zl];
+  const int64_t zq = zl < nzc ? zone_at(zorder, z0 + zl) : 0;
+  const bool valid = zl < nzc && sel_ptr[zq + 1] > sel_ptr[zq];
   if (valid) {
     copy_batched<16>(0, NP, 1, [&](int k) { return dvec[(size_t)k * nzc + zl]; },
                      [&](int k, double x) { d[k][lane] = x; });
@@ -736,7 +772,7 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int64_t* 
 // while the current one is applied.
 // ---------------------------------------------------------------------------
 template <int NP>
-__global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
+__global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0, const int32_t* __restrict__ zorder,
                                                                  const int64_t* __restrict__ sel_ptr,
                                                                  double* __restrict__ qmat,
                                                                  const double2* __restrict__ rlog,
@@ -749,7 +785,7 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0,
   __shared__ double2 buf[2][NP + 12];
   double* Z = Zs + PAD;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int stz = status[zl];
   if (stz & 3) return;  // failed zone: solved again by the fallback
@@ -864,7 +900,7 @@ struct Apply128Lds {
   double hd[2][2][128];   // [0: x, 1: y][group] partial row products
 };
 
-__global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
                                                        const double* __restrict__ mat,
                                                        double* __restrict__ qmat,
                                                        const double2* __restrict__ rlog,
@@ -878,7 +914,7 @@ __global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, int N, const i
   __shared__ Apply128Lds S;
   double* Z = S.z + 8;
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int tid = threadIdx.x;
   const int g = tid >> 7, r = tid & 127, w = tid >> 6, lane = tid & 63;
   if (sel_ptr[z + 1] == sel_ptr[z]) {  // no observations: a = 0
@@ -1166,7 +1202,7 @@ __global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, int N, const i
 // Epilogue from eigenvectors U (rows in global memory) and eigenvalues d.
 // ---------------------------------------------------------------------------
 template <int NP>
-__global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, int N, const int64_t* __restrict__ sel_ptr,
+__global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
                                                     const double* __restrict__ mat,
                                                     double* __restrict__ umat,
                                                     const double* __restrict__ dvec, int nzc,
@@ -1177,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, int N, const int
   __shared__ double col[6][NP];
   __shared__ double part[H][NP];
   const int zl = blockIdx.x;
-  const int64_t z = z0 + zl;
+  const int64_t z = zone_at(zorder, z0 + zl);
   const int tid = threadIdx.x;
   double* V = vec + (size_t)zl * 4 * NP;
   if (sel_ptr[z + 1] == sel_ptr[z]) {
@@ -1260,14 +1296,23 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, int N, const int
 // ---------------------------------------------------------------------------
 template <int NP>
 static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_tridiag<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.mat, A.ymat,
-                     A.dvec, A.evec, A.nzc);
+  bool split = false;
+  if constexpr (NP == 128) split = A.tri_split != 0;
+  if (split) {
+    hipLaunchKernelGGL((k_tridiag<128, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+    hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+  } else {
+    hipLaunchKernelGGL((k_tridiag<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+  }
   if constexpr (NP == 128)
-    hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.sel_ptr, A.ymat);
+    hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr, A.ymat);
   // zones per QL wave: a tuning knob (the rotation log layout groups 64
   // zones whatever the wave width)
 #define OAK_TQL(ZW)                                                                               \
-  hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0, \
+  hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0, A.zorder, \
                      A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err)
   if (A.ql_zw == 8)
     OAK_TQL(8);
@@ -1277,13 +1322,13 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
     OAK_TQL(16);
 #undef OAK_TQL
   if constexpr (NP == 128) {  // the epilogue runs at the end of the apply kernel
-    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr, A.mat,
+    hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr, A.mat,
                        A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,
                        A.vec, A.ampl, A.err);
   } else {
-    hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0,
+    hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0, A.zorder,
                        A.sel_ptr, A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status);
-    hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.N, A.sel_ptr,
+    hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
   }
 }

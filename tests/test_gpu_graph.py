@@ -63,15 +63,67 @@ def test_graph_replays_match_eager(problem):
     ctx.close()
 
 
-@pytest.mark.parametrize("slots,ql", [(1, 8), (2, 32), (4, 16)])
-def test_tuning_knobs_same_bits(problem, slots, ql):
+@pytest.mark.parametrize("slots,ql,order", [(1, 8, 0), (2, 32, 1), (4, 16, 0), (3, 16, 1),
+                                           (1, 8, 1)])
+def test_tuning_knobs_same_bits(problem, slots, ql, order):
     from oak_amd._lib import Context
     ref_ctx = Context(0)
     ref = _run(ref_ctx, problem)
     ref_ctx.close()
     ctx = Context(0)
-    ctx.set_tuning(slots=slots, ql_zones=ql)
+    ctx.set_tuning(slots=slots, ql_zones=ql, order=order)
     np.testing.assert_array_equal(_run(ctx, problem), ref)
+    ctx.close()
+
+
+def test_order_with_graph_and_timing(problem):
+    """The ordered loop under graph replay and with phase timing (one chunk in
+    order on the caller's stream) gives the same bits."""
+    from oak_amd._lib import Context
+    ref_ctx = Context(0)
+    ref = _run(ref_ctx, problem)
+    ref_ctx.close()
+    ctx = Context(0)
+    ctx.set_tuning(order=1, graph=1)
+    for _ in range(3):
+        np.testing.assert_array_equal(_run(ctx, problem), ref)
+    assert ctx.tuning()["order"] == 1
+    ctx.set_tuning(graph=0)
+    ctx.set_timing(True)
+    np.testing.assert_array_equal(_run(ctx, problem), ref)
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def problem128():
+    from oak_amd import synth
+    from oak_amd import assimilation as A
+    P = synth.column_ocean(24, 20, 2, 100, 1500, 60, seed_cfg=701, hlen=4, vlen=1, L=3.0,
+                           cutoff=9.0)
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt)
+    g = A.Geometry(t(P.zone_ptr, torch.int64), t(P.zpos), t(P.corr_len), t(P.max_len), t(P.ox),
+                   t(P.oy))
+    rp, co, va = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    H = (t(rp, torch.int64), t(co, torch.int32), t(va))
+    return P, g, H, t(P.Hshift), t(P.yo), t(P.rmse2), t(P.E.T).t()
+
+
+@pytest.mark.parametrize("tri_split,order,graph", [(1, 0, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)])
+def test_two_launch_reduction_same_bits(problem128, tri_split, order, graph):
+    """N = 100 (NP = 128): the Householder reduction as two kernels, with and
+    without the zone order and graph replay, gives the bits of the default
+    run (one kernel, zones in their order, eager launches)."""
+    from oak_amd._lib import Context
+    assert problem128[6].shape[1] == 100
+    ref_ctx = Context(0)
+    ref = _run(ref_ctx, problem128)
+    ref_ctx.close()
+    ctx = Context(0)
+    ctx.set_tuning(tri_split=tri_split, order=order, graph=graph)
+    for _ in range(2):
+        np.testing.assert_array_equal(_run(ctx, problem128), ref)
+    assert ctx.tuning()["tri_split"] == tri_split
     ctx.close()
 
 
@@ -79,7 +131,8 @@ def test_tuning_rejects_bad_values():
     from oak_amd import OakError
     from oak_amd._lib import Context
     ctx = Context(0)
-    for kw in (dict(slots=0), dict(slots=5), dict(ql_zones=12), dict(graph=2)):
+    for kw in (dict(slots=0), dict(slots=5), dict(ql_zones=12), dict(graph=2), dict(order=2),
+               dict(tri_split=3)):
         with pytest.raises(OakError):
             ctx.set_tuning(**kw)
     ctx.close()

@@ -150,7 +150,7 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
  * computed alone, so every setting gives the same bits):
  *   OAKG_TUNE_SLOTS       zone chunks in flight on as many streams (1..4,
  *                         default 3; from 4096 zones, phase timing off)
- *   OAKG_TUNE_QL_ZONES    zones per wave of the QL kernel (8, 16, 32; 16)
+ *   OAKG_TUNE_QL_ZONES    zones per wave of the QL kernel (4, 8, 16, 32; 16)
  *   OAKG_TUNE_GRAPH       1: the zone loop (Gram, eigensolve, transform,
  *                         update of every chunk, on all its streams) is
  *                         captured in a HIP graph on first use and replayed

@@ -16,7 +16,7 @@ import json
 import os
 import time
 
-CANDIDATES = {"slots": (1, 2, 3, 4), "ql_zones": (8, 16, 32), "graph": (0, 1), "order": (0, 1),
+CANDIDATES = {"slots": (1, 2, 3, 4), "ql_zones": (4, 8, 16, 32), "graph": (0, 1), "order": (0, 1),
               "tri_split": (0, 1)}
 DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0, "order": 0, "tri_split": 0}
 

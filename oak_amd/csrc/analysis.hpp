@@ -93,7 +93,7 @@ struct ChunkArgs {
   int32_t* sweeps;
   // eigensolver: 0 = one-sided Jacobi, 1 = Householder tridiagonalisation + implicit QL
   int eig_method;
-  int ql_zw;                // zones per wave of the QL kernel (8, 16 or 32)
+  int ql_zw;                // zones per wave of the QL kernel (4, 8, 16 or 32)
   int tri_split;            // NP = 128: the Householder reduction as two launches
   double *dvec, *evec;      // tridiagonal / eigenvalues, NP x nzc
   void* rlog;               // QL rotation log, rlog_cap records of 16 B per zone

@@ -1578,7 +1578,7 @@ int oakg_set_tuning(oakg_ctx* c, int key, int value) {
       c->tune_slots = value;
       return OAKG_OK;
     case OAKG_TUNE_QL_ZONES:
-      if (value != 8 && value != 16 && value != 32) break;
+      if (value != 4 && value != 8 && value != 16 && value != 32) break;
       c->tune_ql_zw = value;
       return OAKG_OK;
     case OAKG_TUNE_ORDER:

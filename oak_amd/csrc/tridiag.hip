@@ -7,7 +7,7 @@
 //                  per thread): M = Q T Q^T with Householder reflectors
 //                  (lower), then Q formed explicitly by backward
 //                  accumulation.  Writes Q and (d, e).
-//   k_tql          one lane per zone (16 zones per wave): implicit QL with
+//   k_tql          one lane per zone (4-32 zones per wave): implicit QL with
 //                  Wilkinson shifts on (d, e) (Golub & Van Loan, 4th ed.,
 //                  section 8.3; Bowdler, Martin, Reinsch & Wilkinson 1968);
 //                  the sequential recurrences of many zones run side by side.
@@ -1314,7 +1314,9 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
 #define OAK_TQL(ZW)                                                                               \
   hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0, A.zorder, \
                      A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err)
-  if (A.ql_zw == 8)
+  if (A.ql_zw == 4)
+    OAK_TQL(4);
+  else if (A.ql_zw == 8)
     OAK_TQL(8);
   else if (A.ql_zw == 32)
     OAK_TQL(32);

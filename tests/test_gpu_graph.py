@@ -64,7 +64,7 @@ def test_graph_replays_match_eager(problem):
 
 
 @pytest.mark.parametrize("slots,ql,order", [(1, 8, 0), (2, 32, 1), (4, 16, 0), (3, 16, 1),
-                                           (1, 8, 1)])
+                                           (1, 8, 1), (3, 4, 1)])
 def test_tuning_knobs_same_bits(problem, slots, ql, order):
     from oak_amd._lib import Context
     ref_ctx = Context(0)

@@ -60,10 +60,15 @@ def device_inputs(DP, device):
     return g, H, t(P.Hshift), t(P.yo), t(P.rmse2)
 
 
-def flops_model(N, counts, zone_sizes):
+def flops_model(N, counts, zone_sizes, dual_lt=0):
     """Algorithmic FLOP per analysis, per phase, from the per-zone local
     observation counts p_z (zones with p_z > 0), counted as the kernels
-    compute them:
+    compute them; with the dual bucket on (N > 64, 1 <= p_z < dual_lt) those
+    zones count the p x p path instead of the Gram and the eigensolve:
+      gram   2 PD^2 N                 (G = S~ S~^T, PD = 64 padded rows)
+      eig    9 PD^3 + 2 N PD p + 4 N p (the 64 x 64 eigensolve and the lift
+                                       W = S~^T U c, a, M^1/2 1)
+    and otherwise:
       gram   2 N^2 p + 2 N p          (S^T R^-1 S and S^T R^-1 d)
       eig    9 N^3                    (symmetric eigensolve with vectors,
                                        Golub & Van Loan's count for
@@ -75,12 +80,17 @@ def flops_model(N, counts, zone_sizes):
                                        would be 6 N^3)
       update 2 s N^2 + 2 s N          (Sf_z [T | a])"""
     act = counts > 0
-    p = counts[act].astype(np.float64)
+    dual = act & (counts < (dual_lt if N > 64 else 0))
+    prim = act & ~dual
+    p = counts[prim].astype(np.float64)
+    pd = counts[dual].astype(np.float64)
     s = zone_sizes[act].astype(np.float64)
     nact = float(act.sum())
+    PD = 64.0
     return {
-        "gram": float(np.sum(2.0 * N * N * p + 2.0 * N * p)),
-        "eig": 9.0 * N ** 3 * nact,
+        "gram": float(np.sum(2.0 * N * N * p + 2.0 * N * p)) + 2.0 * PD * PD * N * len(pd),
+        "eig": 9.0 * N ** 3 * float(prim.sum())
+               + float(np.sum(9.0 * PD ** 3 + 2.0 * N * PD * pd + 4.0 * N * pd)),
         "trans": (N * N * (N + 1.0) + 10.0 * N * N) * nact,
         "update": float(np.sum(2.0 * s * N * N + 2.0 * s * N)),
     }
@@ -405,6 +415,14 @@ def run_config3(args, device):
     torch.cuda.synchronize()
     t_adam = time.perf_counter() - t1
     sig = torch.exp(ls).cpu().numpy()
+    # the same with one sigma per class of observations (even / odd indices,
+    # generated with 0.05 / 0.2): what one innovation vector can identify
+    cls = np.arange(m) % 2
+    lsc, L2, a2, hist2 = tune(prob, torch.full((2,), float(np.log(0.3)), dtype=torch.float64),
+                              L0, a0, steps=200, lr=0.05, classes=cls)
+    sigc = torch.exp(lsc).cpu().numpy()
+    print("# config3: tied sigma per class %.4f / %.4f (generated 0.05 / 0.2), L %.2f, "
+          "inflation %.3f" % (sigc[0], sigc[1], L2, a2), file=sys.stderr, flush=True)
     out = {
         "metric": "analysed state elems/s",
         "value": P.n / (ms * 1e-3),
@@ -420,6 +438,10 @@ def run_config3(args, device):
                  "sigma_start": 0.3,
                  "sigma_mean_true_0.05": float(sig[0::2].mean()),
                  "sigma_mean_true_0.2": float(sig[1::2].mean())},
+        "adam_tied_sigma": {"steps": 200, "lr": 0.05, "classes": "even/odd observation index",
+                            "J_first": hist2[0], "J_last": hist2[-1], "L": [L0, L2],
+                            "inflation": [a0, a2], "sigma_true_0.05": float(sigc[0]),
+                            "sigma_true_0.2": float(sigc[1])},
     }
     print(json.dumps(out), flush=True)
 
@@ -500,9 +522,9 @@ def main():
         ctx.close()
         del E32
     head = "mixed" if "mixed" in res else "fp64"
-    ms, phase, _ = res[head]
+    ms, phase, st_head = res[head]
     zs = np.diff(P.zone_ptr)
-    F = flops_model(N, counts, zs)
+    F = flops_model(N, counts, zs, st_head.get("tuning", {}).get("dual", 0))
     out = {
         "metric": "analysed state elems/s",
         "value": n / (ms * 1e-3),
@@ -522,10 +544,12 @@ def main():
         "launch": res[head][2].get("tuning"),
     }
     if "mixed" in res and "fp64" in res:
-        ms64, ph64, _ = res["fp64"]
+        ms64, ph64, st64 = res["fp64"]
+        F64 = flops_model(N, counts, zs, st64.get("tuning", {}).get("dual", 0))
         out["fp64"] = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64,
                        "phase_ms": {k: round(v, 3) for k, v in ph64.items()},
-                       "roofline_phases": phase_rooflines(F, ph64, False)}
+                       "roofline_phases": phase_rooflines(F64, ph64, False),
+                       "launch": st64.get("tuning")}
         out["mixed_speedup_vs_fp64"] = ms64 / ms
         e = ensemble_errors(Ea32, Ea64, DP.E)
         out["mixed_err_vs_fp64"] = dict(e, bound=1e-4)

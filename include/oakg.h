@@ -146,8 +146,9 @@ int oakg_set_timing(oakg_ctx* ctx, int on);
 #define OAKG_EIG_QL 1
 int oakg_set_eigensolver(oakg_ctx* ctx, int method);
 int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
-/* Tuning knobs of the zone loop (results do not depend on them; each zone is
- * computed alone, so every setting gives the same bits):
+/* Tuning knobs of the zone loop (each zone is computed alone, so every
+ * setting but OAKG_TUNE_DUAL and OAKG_TUNE_TRIDIAG_SPLIT = 2 gives the same
+ * bits; those two give the same analysis to rounding):
  *   OAKG_TUNE_SLOTS       zone chunks in flight on as many streams (1..4,
  *                         default 3; from 4096 zones, phase timing off)
  *   OAKG_TUNE_QL_ZONES    zones per wave of the QL kernel (4, 8, 16, 32; 16)
@@ -164,7 +165,15 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
  *   OAKG_TUNE_TRIDIAG_SPLIT  1: at N > 64 the Householder reduction runs as
  *                         two kernels (the first half of the steps, then the
  *                         rest with fewer registers and more zones per CU);
+ *                         2: three kernels, the middle quarter of the steps
+ *                         on 6 x 6 register tiles (sums grouped differently);
  *                         0: one kernel (default)
+ *   OAKG_TUNE_DUAL        at N > 64, zones with 1 <= p < value local
+ *                         observations (value 16, 32 or 64; 0 = off, the
+ *                         default) are solved in the p x p observation space:
+ *                         G = S~ S~^T instead of M = I + S~^T S~, which have
+ *                         the same nonzero spectrum.  The analysis is the
+ *                         same to rounding (not bit for bit).
  * oakg_get_tuning also reads OAKG_TUNE_GRAPH_REPLAYS / _CAPTURES (counts). */
 #define OAKG_TUNE_SLOTS 1
 #define OAKG_TUNE_QL_ZONES 2
@@ -173,6 +182,7 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
 #define OAKG_TUNE_GRAPH_CAPTURES 5
 #define OAKG_TUNE_ORDER 6
 #define OAKG_TUNE_TRIDIAG_SPLIT 7
+#define OAKG_TUNE_DUAL 8
 int oakg_set_tuning(oakg_ctx* ctx, int key, int value);
 int oakg_get_tuning(oakg_ctx* ctx, int key, int64_t* value);
 /* Per-zone local observation counts p_z of the last call (host array). */

@@ -31,7 +31,7 @@ EXPORTS = [
 ]
 OAKG_TUNE_SLOTS, OAKG_TUNE_QL_ZONES, OAKG_TUNE_GRAPH = 1, 2, 3
 OAKG_TUNE_GRAPH_REPLAYS, OAKG_TUNE_GRAPH_CAPTURES, OAKG_TUNE_ORDER = 4, 5, 6
-OAKG_TUNE_TRIDIAG_SPLIT = 7
+OAKG_TUNE_TRIDIAG_SPLIT, OAKG_TUNE_DUAL = 7, 8
 
 
 class OakError(RuntimeError):
@@ -194,14 +194,16 @@ class Context:
         """OAKG_EIG_QL (default) or OAKG_EIG_JACOBI."""
         self.check(load().oakg_set_eigensolver(self._h, int(method)))
 
-    def set_tuning(self, slots=None, ql_zones=None, graph=None, order=None, tri_split=None):
+    def set_tuning(self, slots=None, ql_zones=None, graph=None, order=None, tri_split=None,
+                   dual=None):
         """Zone chunks in flight, zones per QL wave, HIP-graph replay of the
         zone loop, zones ordered by observation count, the N > 64 Householder
-        reduction as two kernels (oakg_set_tuning)."""
+        reduction as two kernels, zones with fewer than `dual` observations
+        solved in observation space (oakg_set_tuning)."""
         L = load()
         for key, v in ((OAKG_TUNE_SLOTS, slots), (OAKG_TUNE_QL_ZONES, ql_zones),
                        (OAKG_TUNE_GRAPH, graph), (OAKG_TUNE_ORDER, order),
-                       (OAKG_TUNE_TRIDIAG_SPLIT, tri_split)):
+                       (OAKG_TUNE_TRIDIAG_SPLIT, tri_split), (OAKG_TUNE_DUAL, dual)):
             if v is not None:
                 self.check(L.oakg_set_tuning(self._h, key, int(v)))
 
@@ -210,7 +212,7 @@ class Context:
         out = {}
         for name, key in (("slots", OAKG_TUNE_SLOTS), ("ql_zones", OAKG_TUNE_QL_ZONES),
                           ("graph", OAKG_TUNE_GRAPH), ("order", OAKG_TUNE_ORDER),
-                          ("tri_split", OAKG_TUNE_TRIDIAG_SPLIT),
+                          ("tri_split", OAKG_TUNE_TRIDIAG_SPLIT), ("dual", OAKG_TUNE_DUAL),
                           ("graph_replays", OAKG_TUNE_GRAPH_REPLAYS),
                           ("graph_captures", OAKG_TUNE_GRAPH_CAPTURES)):
             v = C.c_int64(0)

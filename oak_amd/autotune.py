@@ -1,24 +1,29 @@
 """Autotuner of the zone loop's launch shapes.
 
-The knobs (oakg_set_tuning) change how the zones are scheduled, never the
-results: the number of zone chunks in flight on as many streams, the number
-of zones per wave of the QL kernel, whether the zone loop is replayed from a
-HIP graph, whether the zones run in order of decreasing observation count
-(the heaviest first, similar zones side by side in a QL wave), and whether
-the N > 64 Householder reduction runs as two kernels (the second with fewer
-registers and more zones per CU).  ``tune`` times every candidate on the caller's own
-workload (device-resident inputs, the same call repeated), keeps the
-fastest, and returns a table entry; ``apply`` sets a context from a table
-(a JSON file keyed by ensemble size and precision, with the workload it was
-tuned on)."""
+The knobs (oakg_set_tuning) change how the zones are scheduled:
+  slots      zone chunks in flight on as many streams,
+  ql_zones   zones per wave of the QL kernel,
+  graph      the zone loop replayed from a HIP graph,
+  order      zones in order of decreasing observation count (the heaviest
+             first, similar zones side by side in a QL wave),
+  tri_split  the N > 64 Householder reduction as two kernels (the second
+             with fewer registers and more zones per CU) or three (the
+             middle one on 6 x 6 register tiles),
+  dual       the bucket bound below which a zone is solved in its p x p
+             observation space.
+All but dual and tri_split = 2 give the same bits; those give the same
+analysis to rounding.  ``tune`` times every candidate on the caller's own workload
+(device-resident inputs, the same call repeated), keeps the fastest, and
+returns a table entry; ``apply`` sets a context from a table (a JSON file
+keyed by ensemble size and precision, with the workload it was tuned on)."""
 import itertools
 import json
 import os
 import time
 
-CANDIDATES = {"slots": (1, 2, 3, 4), "ql_zones": (4, 8, 16, 32), "graph": (0, 1), "order": (0, 1),
-              "tri_split": (0, 1)}
-DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0, "order": 0, "tri_split": 0}
+CANDIDATES = {"slots": (2, 3, 4), "ql_zones": (4, 8, 16), "graph": (0, 1), "order": (0, 1),
+              "tri_split": (0, 1, 2), "dual": (0, 32, 64)}
+DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0, "order": 0, "tri_split": 0, "dual": 0}
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 

@@ -317,8 +317,15 @@ __global__ __launch_bounds__(256) void k_eig(int64_t z0, const int32_t* __restri
   const int grp = lane / NP, r = lane % NP, gb = grp * NP;
   const int zl = (blockIdx.x * 4 + wave) * GPW + grp;
   // with `only`, zones whose flag has no bit in 3 are left untouched
-  const bool valid = zl < nzc && (!only || (only[zl] & 3));
-  const int64_t z = zone_at(zorder, z0 + (valid ? zl : 0));
+  bool valid = zl < nzc && (!only || (only[zl] & 3));
+  int64_t z = valid ? zone_at(zorder, z0 + zl) : 0;
+  if (z < 0) {  // solved by the other pipeline
+    valid = false;
+    z = 0;
+  }
+  // a wave with no zone to solve leaves at once (the fallback pass visits
+  // every zone and redoes only the flagged ones); the waves share no barrier
+  if (__ballot(valid) == 0) return;
   const bool active = valid && (sel_ptr[z + 1] > sel_ptr[z]);
   const double* M = mat + (size_t)(valid ? zl : 0) * NP * NP;
   double x[NP];
@@ -657,17 +664,11 @@ void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st) {
   }
 }
 
+// The eigensolver of a chunk (M in A.mat, b in A.vec) for NP <= 64: a, v,
+// mu into A.vec and Y = U diag(mu^-1/4) into A.ymat.
 template <int NP>
-static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
+static void eig_t(const ChunkArgs& A, hipStream_t st) {
   constexpr int GPW = 64 / NP;
-  if (tm) tm->mark(PH_GRAM);
-  if (A.prec == 1)
-    hipLaunchKernelGGL((k_gram<NP, float>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
-  else
-    hipLaunchKernelGGL((k_gram<NP, double>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
-  if (tm) tm->mark(PH_EIG);
   if (A.eig_method == 1) {
     launch_tql_pipeline(NP, A, st);
     // zones the QL could not finish fall back to Jacobi
@@ -679,6 +680,27 @@ static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
                        A.N, A.sel_ptr, A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol,
                        A.max_sweeps, (const int32_t*)nullptr);
   }
+}
+
+void launch_eig_small(int NP, const ChunkArgs& A, hipStream_t st) {
+  switch (NP) {
+    case 16: eig_t<16>(A, st); break;
+    case 32: eig_t<32>(A, st); break;
+    case 64: eig_t<64>(A, st); break;
+  }
+}
+
+template <int NP>
+static void chunk_t(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
+  if (tm) tm->mark(PH_GRAM);
+  if (A.prec == 1)
+    hipLaunchKernelGGL((k_gram<NP, float>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
+  else
+    hipLaunchKernelGGL((k_gram<NP, double>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec, A.err);
+  if (tm) tm->mark(PH_EIG);
+  eig_t<NP>(A, st);
   if (A.mode == 2) {  // innovation likelihood: adjoint instead of transform + update
     if (tm) tm->mark(PH_UPDATE);
     launch_nll_adjoint(NP, A, st);

@@ -147,6 +147,16 @@ struct ReportArgs {
 void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st);
 void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+// NP = 128 in two parts (Gram + eigensolve, then transform + update), so
+// that the zones solved in observation space can run in between
+void launch_solve128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+void launch_apply128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+// eigensolver of a chunk for NP <= 64 (QL pipeline or Jacobi, with fallback)
+void launch_eig_small(int NP, const ChunkArgs& A, hipStream_t st);
+// zones with 1 <= p < 64 at NP = 128 in observation space (dual.hip): A has
+// the dual order and the NP = 64 buffers; ymat128/vec128/ampl receive W, a, v
+void launch_dual128(const ChunkArgs& A, int N, double* ymat128, double* vec128, double* ampl,
+                    hipStream_t st, PhaseTimer* tm);
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int32_t* __re
                                                  unsigned* __restrict__ err) {
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   const int64_t p0 = sel_ptr[z];
   const int64_t p = sel_ptr[z + 1] - p0;
   double* M = mat + (size_t)zl * P128 * P128;
@@ -204,6 +205,7 @@ __global__ __launch_bounds__(256) void k_eig128(int64_t z0, const int32_t* __res
   __shared__ Eig128Lds L;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   if (only && !(only[zl] & 3)) return;
   double* V = vec + (size_t)zl * 4 * P128;
@@ -519,6 +521,9 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, const int32_t* __r
     }
   };
   const double* Vz = vec + (size_t)zl * 4 * P128;
+  // a zone solved in observation space (dual.hip) has -1 in its mu slot and
+  // W in place of Y: M^-1/2 = I - W W^T instead of Y Y^T
+  const bool dual = Vz[3 * P128] < 0.0;
   if (tid < P128) L.a[tid] = Vz[P128 + tid];
   else if (tid < 2 * P128) L.v[tid - P128] = tid - P128 < N ? Vz[2 * P128 + tid - P128] : 0.0;
   if (sel_ptr[z + 1] == sel_ptr[z]) {
@@ -580,8 +585,9 @@ __global__ __launch_bounds__(512) void k_xform128(int64_t z0, const int32_t* __r
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int m = 16 * I + kk + 4 * r, n = 16 * J + li;
-          L.t[m * LDT + n] = acc[r];
-          L.t[n * LDT + m] = acc[r];
+          const double tv = dual ? (m == n ? 1.0 : 0.0) - acc[r] : acc[r];
+          L.t[m * LDT + n] = tv;
+          L.t[n * LDT + m] = tv;
         }
       };
       if (!hi) {
@@ -767,6 +773,7 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, const int32_t* __
     }
   };
   const double* Vz = vec + (size_t)zl * 4 * P128;
+  const bool dual = Vz[3 * P128] < 0.0;  // T = (I - W W^T) Omega (dual.hip)
   if (tid < P128) L.a[tid] = Vz[P128 + tid] / scaling;
   else L.v[tid - P128] = tid - P128 < N ? Vz[2 * P128 + tid - P128] : 0.0;
   if (sel_ptr[z + 1] == sel_ptr[z]) {
@@ -826,8 +833,9 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, const int32_t* __
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int m = 16 * I + 4 * kk + r, n = 16 * J + li;
-          L.t[m * LDF + n] = acc[r];
-          L.t[n * LDF + m] = acc[r];
+          const float tv = dual ? (m == n ? 1.0f : 0.0f) - acc[r] : acc[r];
+          L.t[m * LDF + n] = tv;
+          L.t[n * LDF + m] = tv;
         }
       };
 #pragma unroll
@@ -943,7 +951,7 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, const int32_t* __
   }
 }
 
-void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
+void launch_solve128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_GRAM);
   if (A.prec == 1)
     hipLaunchKernelGGL(k_gram128<float>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
@@ -962,6 +970,9 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
                        A.mat, A.ymat, A.vec, A.ampl, A.sweeps, A.err, A.tol, A.max_sweeps,
                        (const int32_t*)nullptr);
   }
+}
+
+void launch_apply128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (A.mode == 2) {  // innovation likelihood: adjoint instead of transform + update
     if (tm) tm->mark(PH_UPDATE);
     launch_nll_adjoint(128, A, st);
@@ -983,6 +994,11 @@ void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
                        A.zone_ptr, A.ymat, A.vec, A.X0, A.S, A.ldS, A.xa, A.Out, A.ldO, A.scaling,
                        A.infl, A.maxcorr, A.maxcorr_v);
   if (tm) tm->mark(PH_END);
+}
+
+void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
+  launch_solve128(A, st, tm);
+  launch_apply128(A, st, tm);
 }
 
 }  // namespace oakg

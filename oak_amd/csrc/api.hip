@@ -69,6 +69,7 @@ struct oakg_ctx {
   int tune_ql_zw = 16;
   int tune_order = 0;  // 1: zones ordered by observation count (k_zone_order)
   int tune_tri_split = 0;  // 1: N = 128 Householder reduction as two launches
+  int tune_dual = 0;  // N = 128: zones with 1 <= p < tune_dual solved in observation space
   // the zone loop captured in a HIP graph and replayed while its launches
   // (kernel arguments, buffers, streams) stay the same
   bool graph = false;
@@ -192,6 +193,20 @@ __global__ __launch_bounds__(1024) void k_zone_order(int64_t nz, int64_t K,
     const int64_t ch = i % K, j = i / K;
     zorder[ch * q + min(ch, r) + j] = (int32_t)z;
   }
+}
+
+// The two sides of the dual split (run_zones): position -> zone for the
+// zones of each side, -1 at the other side's positions.
+__global__ void k_dual_split(int64_t nz, const int32_t* __restrict__ zorder,
+                             const int64_t* __restrict__ sel_ptr, int dual_lt,
+                             int32_t* __restrict__ zord_p, int32_t* __restrict__ zord_d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nz) return;
+  const int32_t z = zorder ? zorder[i] : (int32_t)i;
+  const int64_t p = sel_ptr[z + 1] - sel_ptr[z];
+  const bool dual = p >= 1 && p < dual_lt;
+  zord_p[i] = dual ? -1 : z;
+  zord_d[i] = dual ? z : -1;
 }
 
 void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st) {
@@ -443,8 +458,13 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   // (about 1.1 N^2 records for typical zones; 2 N^2 + 8 N leaves room for slow ones)
   const int64_t cap = (((int64_t)2 * NP * NP + 8 * NP) + 3) & ~(int64_t)3;
   const bool ql = c->eig_method == 1;
+  // zones with 1 <= p < dual_lt in observation space (dual.hip): N > 64, the
+  // analysis modes (not the likelihood), not in a no-allocation call
+  const int dual_lt = (NP == 128 && base.mode != 2 && !c->arena) ? c->tune_dual : 0;
+  constexpr int PD = 64;
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
-                          (ql ? (size_t)cap * 16 + 4 : 0);
+                          (ql ? (size_t)cap * 16 + 4 : 0) +
+                          (dual_lt ? sizeof(double) * ((size_t)2 * PD * PD + 4 * PD) : 0);
   const size_t budget = c->ws_budget;
   const int nslot = (!c->timing && !c->arena && nz >= 4096) ? c->tune_slots : 1;
   int64_t zc = std::max<int64_t>(1, (int64_t)(budget / (per_zone * nslot)));
@@ -486,6 +506,19 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     zorder = (int32_t*)wsget(c, Workspace::ZORDER, sizeof(int32_t) * (size_t)nz);
     if (!zorder) check(c, hipErrorOutOfMemory, "workspace");
   }
+  // the dual split: the loop's order with -1 at the zones of the other side,
+  // and the NP = 64 buffers of the dual side
+  int32_t *zord_p = nullptr, *zord_d = nullptr;
+  double *mat_d = nullptr, *ymat_d = nullptr, *vec_d = nullptr;
+  if (dual_lt) {
+    if (nz > INT32_MAX) arg_fail(c, "more than 2^31-1 zones");
+    zord_p = (int32_t*)wsget(c, Workspace::ZORDER_P, sizeof(int32_t) * (size_t)nz);
+    zord_d = (int32_t*)wsget(c, Workspace::ZORDER_D, sizeof(int32_t) * (size_t)nz);
+    mat_d = (double*)wsget(c, Workspace::MAT_D, sizeof(double) * (size_t)PD * PD * zs);
+    ymat_d = (double*)wsget(c, Workspace::YMAT_D, sizeof(double) * (size_t)PD * PD * zs);
+    vec_d = (double*)wsget(c, Workspace::VEC_D, sizeof(double) * (size_t)4 * PD * zs);
+    if (!zord_p || !zord_d || !mat_d || !ymat_d || !vec_d) check(c, hipErrorOutOfMemory, "workspace");
+  }
   // streams and events of the pipeline, created before any capture
   if (nslot > 1) {
     if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
@@ -505,8 +538,8 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     struct Key {
       ChunkArgs base;
       int64_t nz, zc, cap;
-      int NP, nslot, ql, ql_zw, order, tri_split;
-      const void* buf[9];
+      int NP, nslot, ql, ql_zw, order, tri_split, dual_lt;
+      const void* buf[14];
       hipStream_t stream;
     } kk;
     memset(&kk, 0, sizeof kk);
@@ -520,7 +553,9 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     kk.ql_zw = c->tune_ql_zw;
     kk.order = order;
     kk.tri_split = c->tune_tri_split;
-    const void* bufs[9] = {mat, ymat, vec, dvec, evec, rlog, tst, sweeps_all, zorder};
+    kk.dual_lt = dual_lt;
+    const void* bufs[14] = {mat,  ymat,   vec,    dvec,   evec,  rlog,  tst,
+                            sweeps_all, zorder, zord_p, zord_d, mat_d, ymat_d, vec_d};
     memcpy(kk.buf, bufs, sizeof bufs);
     kk.stream = c->stream;
     key.assign((const char*)&kk, (const char*)&kk + sizeof kk);
@@ -556,6 +591,11 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     hipLaunchKernelGGL(k_zone_order, dim3(1), dim3(1024), 0, s0, nz, nchunk, base.sel_ptr, zorder);
     check(c, hipGetLastError(), "zone order");
   }
+  if (dual_lt) {
+    hipLaunchKernelGGL(k_dual_split, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s0, nz, zorder,
+                       base.sel_ptr, dual_lt, zord_p, zord_d);
+    check(c, hipGetLastError(), "dual split");
+  }
   hipStream_t S[OAKG_PIPE_SLOTS];
   for (int i = 0; i < OAKG_PIPE_SLOTS; ++i) S[i] = s0;
   if (nslot > 1) {
@@ -588,7 +628,26 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     A.tql_status = ql ? tst + zo : nullptr;
     A.ql_zw = c->tune_ql_zw;
     A.tri_split = c->tune_tri_split;
-    launch_chunk(NP, A, S[sl], c->timing ? &c->timer : nullptr);
+    PhaseTimer* tm = c->timing ? &c->timer : nullptr;
+    if (dual_lt) {
+      // ensemble-space zones, then observation-space zones (writing their
+      // transform inputs into the same slots), then the transform and update
+      // of every zone of the chunk
+      ChunkArgs Ap = A;
+      Ap.zorder = zord_p;
+      launch_solve128(Ap, S[sl], tm);
+      ChunkArgs Ad = A;
+      Ad.zorder = zord_d;
+      Ad.mat = mat_d + zo * PD * PD;
+      Ad.ymat = ymat_d + zo * PD * PD;
+      Ad.vec = vec_d + zo * 4 * PD;
+      Ad.ampl = nullptr;
+      Ad.N = PD;
+      launch_dual128(Ad, A.N, A.ymat, A.vec, A.ampl, S[sl], tm);
+      launch_apply128(A, S[sl], tm);
+    } else {
+      launch_chunk(NP, A, S[sl], tm);
+    }
     check(c, hipGetLastError(), "chunk launch");
   }
   for (int i = 1; i < nslot; ++i) {
@@ -1586,8 +1645,12 @@ int oakg_set_tuning(oakg_ctx* c, int key, int value) {
       c->tune_order = value;
       return OAKG_OK;
     case OAKG_TUNE_TRIDIAG_SPLIT:
-      if (value != 0 && value != 1) break;
+      if (value < 0 || value > 2) break;
       c->tune_tri_split = value;
+      return OAKG_OK;
+    case OAKG_TUNE_DUAL:
+      if (value != 0 && value != 16 && value != 32 && value != 64) break;
+      c->tune_dual = value;
       return OAKG_OK;
     case OAKG_TUNE_GRAPH:
       if (value != 0 && value != 1) break;
@@ -1613,6 +1676,7 @@ int oakg_get_tuning(oakg_ctx* c, int key, int64_t* value) {
     case OAKG_TUNE_GRAPH: *value = c->graph ? 1 : 0; return OAKG_OK;
     case OAKG_TUNE_ORDER: *value = c->tune_order; return OAKG_OK;
     case OAKG_TUNE_TRIDIAG_SPLIT: *value = c->tune_tri_split; return OAKG_OK;
+    case OAKG_TUNE_DUAL: *value = c->tune_dual; return OAKG_OK;
     case OAKG_TUNE_GRAPH_REPLAYS: *value = c->graph_replays; return OAKG_OK;
     case OAKG_TUNE_GRAPH_CAPTURES: *value = c->graph_captures; return OAKG_OK;
     default: break;

@@ -112,10 +112,12 @@ __device__ __forceinline__ void ring_tile(int t, int& tr, int& tc) {
 // launches, PART 1 the first stage (the 8 x 8 tiles that need most of the
 // registers) ending with the trailing half-block [H2, NP)^2 stored in the
 // rows H2 .. NP-1 of qmat (not yet holding reflectors: row r, column c - H2),
-// then PART 2 the later stages from that block.  PART 2 needs a fraction of
-// the registers, so more zones share a CU through its latency-bound steps.
-// Both ways run the same operations on the same values: bit-identical T
-// and reflectors.
+// then PART 2 the later stages from that block; or as three, PART 3 (steps
+// 0 .. 31), PART 4 (steps 32 .. 63 on 6 x 6 tiles of the 96 x 96 trailing
+// block) and PART 2.  The later launches need a fraction of the registers,
+// so more zones share a CU through their latency-bound steps.  Every way
+// runs the same operations on the same values: bit-identical T and
+// reflectors.
 template <int NP, int PART>
 __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                                  const double* __restrict__ mat,
@@ -129,9 +131,10 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
   constexpr int T2 = T1 >= 2 ? T1 / 2 : T1;
   constexpr int T3 = T2 >= 2 ? T2 / 2 : T2;
   constexpr int H2 = NP - 16 * T2, H3 = NP - 16 * T3;
-  __shared__ TriLds<NP, PART == 2 ? NP / 4 : NP / 2> S;
+  __shared__ TriLds<NP, PART == 2 ? NP / 4 : (PART >= 3 ? 1 : NP / 2)> S;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   int tr, tc;
@@ -157,8 +160,12 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
   double* col = S.colw[wave];
   auto reduce_steps = [&](auto& a, auto Tk, int r0, int c0, int base, int i0, int i1) {
     constexpr int T = decltype(Tk)::value;
-    // a tile's T rows (columns) lie in one group of 8: tri_rx(r0 + ii) = R0 + ii
+    // the padded index of a tile's row (column) ii: for T dividing 8 the T
+    // rows lie in one group of 8, tri_rx(r0 + ii) = tri_rx(r0) + ii; tiles of
+    // 6 can straddle a group
     const int R0 = tri_rx(r0), C0 = tri_rx(c0);
+    auto RX = [&](int ii) { return 8 % T == 0 ? R0 + ii : tri_rx(r0 + ii); };
+    auto CX = [&](int jj) { return 8 % T == 0 ? C0 + jj : tri_rx(c0 + jj); };
     for (int i = i0; i < i1; ++i) {
       const int b = i & 1;
       double* pw = S.pw[b];
@@ -226,9 +233,9 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
       const bool act = (r0 + T - 1 > i) && (c0 + T - 1 > i);
       double vR[T], vC[T];
 #pragma unroll
-      for (int jj = 0; jj < T; ++jj) vC[jj] = col[C0 + jj];
+      for (int jj = 0; jj < T; ++jj) vC[jj] = col[CX(jj)];
 #pragma unroll
-      for (int ii = 0; ii < T; ++ii) vR[ii] = col[R0 + ii];
+      for (int ii = 0; ii < T; ++ii) vR[ii] = col[RX(ii)];
 #pragma unroll
       for (int ii = 0; ii < T; ++ii) {
         double ps = 0;
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
 #pragma unroll
           for (int jj = 0; jj < T; ++jj) ps = fma(a[ii][jj], vC[jj], ps);
         }
-        S.part[tc][R0 + ii] = ps;
+        S.part[tc][RX(ii)] = ps;
       }
       // column i+1 before this step's update, from the threads of its tile column
       if (tc == (i + 1 - base) / T) {
@@ -245,7 +252,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
         for (int jj = 0; jj < T; ++jj)
           if (jj == j0) {
 #pragma unroll
-            for (int ii = 0; ii < T; ++ii) pub[R0 + ii] = a[ii][jj];
+            for (int ii = 0; ii < T; ++ii) pub[RX(ii)] = a[ii][jj];
           }
       }
       lds_barrier();
@@ -278,9 +285,9 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
       if (act) {
         double wR[T], wC[T];
 #pragma unroll
-        for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[R0 + ii]);
+        for (int ii = 0; ii < T; ++ii) wR[ii] = fma(K, vR[ii], pw[RX(ii)]);
 #pragma unroll
-        for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[C0 + jj]);
+        for (int jj = 0; jj < T; ++jj) wC[jj] = fma(K, vC[jj], pw[CX(jj)]);
 #pragma unroll
         for (int ii = 0; ii < T; ++ii)
 #pragma unroll
@@ -380,7 +387,35 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
   using K2 = std::integral_constant<int, T2>;
   using K3 = std::integral_constant<int, T3>;
 
-  static_assert(PART == 0 || (NP == 128 && T2 < T1), "the two-launch reduction is for NP = 128");
+  static_assert(PART == 0 || (NP == 128 && T2 < T1), "the staged launches are for NP = 128");
+  // three launches: steps 0 .. 31 on the 8 x 8 tiles, ending with the block
+  // [32, NP)^2 in rows 32 .. NP-1 of qmat (PART 3); steps 32 .. H2-1 on 6 x 6
+  // tiles of that block (PART 4, 16 x 6 = 96), ending with [H2, NP)^2 as
+  // PART 1 leaves it; then PART 2
+  constexpr int H6 = NP - 16 * 6;
+  if constexpr (PART == 4) {
+    using K6 = std::integral_constant<int, 6>;
+    const int r6 = H6 + tr * 6, c6 = H6 + tc * 6;
+    double a6[6][6];
+#pragma unroll
+    for (int ii = 0; ii < 6; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < 6; ++jj) a6[ii][jj] = V[(size_t)(r6 + ii) * NP + (c6 + jj - H6)];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const int r = lane + 64 * h;
+      x[h] = (r >= H6 && r < NP) ? V[(size_t)r * NP] : 0.0;
+    }
+    __syncthreads();  // every read of the block ends before row H6 gets its reflector
+    reduce_steps(a6, K6{}, r6, c6, H6, H6, H2);
+#pragma unroll
+    for (int ii = 0; ii < 6; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < 6; ++jj)
+        if (r6 + ii >= H2 && c6 + jj >= H2)
+          V[(size_t)(r6 + ii) * NP + (c6 + jj - H2)] = a6[ii][jj];
+    return;
+  }
   // ---- reduction ----
   double a3[T3][T3];
   {
@@ -408,6 +443,16 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
       const int r = lane + 64 * h;
       x[h] = r < NP ? M[(size_t)r * NP] : 0.0;
     }
+      if constexpr (PART == 3) {
+        reduce_steps(a, K1{}, r1, c1, 0, 0, H6);
+#pragma unroll
+        for (int ii = 0; ii < T1; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < T1; ++jj)
+            if (r1 + ii >= H6 && c1 + jj >= H6)
+              V[(size_t)(r1 + ii) * NP + (c1 + jj - H6)] = a[ii][jj];
+        return;
+      }
       if constexpr (T2 < T1) {
         reduce_steps(a, K1{}, r1, c1, 0, 0, H2);
         if constexpr (PART == 1) {
@@ -510,6 +555,7 @@ __global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int32_t* 
   __shared__ FormQLds S;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int lr = lane >> 4, lc = lane & 15;
@@ -658,8 +704,10 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
   const int lane = threadIdx.x;
   if (lane >= ZW) return;
   const int zl = blockIdx.x * ZW + lane;
-  const int64_t zq = zl < nzc ? zone_at(zorder, z0 + zl) : 0;
-  const bool valid = zl < nzc && sel_ptr[zq + 1] > sel_ptr[zq];
+  int64_t zq = zl < nzc ? zone_at(zorder, z0 + zl) : 0;
+  const bool own = zq >= 0;  // negative: solved by the other pipeline
+  if (!own) zq = 0;
+  const bool valid = zl < nzc && own && sel_ptr[zq + 1] > sel_ptr[zq];
   if (valid) {
     copy_batched<16>(0, NP, 1, [&](int k) { return dvec[(size_t)k * nzc + zl]; },
                      [&](int k, double x) { d[k][lane] = x; });
@@ -786,6 +834,7 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0, con
   double* Z = Zs + PAD;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int stz = status[zl];
   if (stz & 3) return;  // failed zone: solved again by the fallback
@@ -915,6 +964,7 @@ __global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, const int32_t*
   double* Z = S.z + 8;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   const int tid = threadIdx.x;
   const int g = tid >> 7, r = tid & 127, w = tid >> 6, lane = tid & 63;
   if (sel_ptr[z + 1] == sel_ptr[z]) {  // no observations: a = 0
@@ -1214,6 +1264,7 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, const int32_t* _
   __shared__ double part[H][NP];
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
   const int tid = threadIdx.x;
   double* V = vec + (size_t)zl * 4 * NP;
   if (sel_ptr[z + 1] == sel_ptr[z]) {
@@ -1296,9 +1347,16 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, const int32_t* _
 // ---------------------------------------------------------------------------
 template <int NP>
 static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
-  bool split = false;
-  if constexpr (NP == 128) split = A.tri_split != 0;
-  if (split) {
+  int split = 0;
+  if constexpr (NP == 128) split = A.tri_split;
+  if (split == 2) {
+    hipLaunchKernelGGL((k_tridiag<128, 3>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+    hipLaunchKernelGGL((k_tridiag<128, 4>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+    hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+  } else if (split == 1) {
     hipLaunchKernelGGL((k_tridiag<128, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc);
     hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,

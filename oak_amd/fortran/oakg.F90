@@ -12,6 +12,9 @@ module oakg
   integer(c_int), parameter :: OAKG_FP64 = 0, OAKG_MIXED = 1
   integer(c_int32_t), parameter :: OAKG_DEVICE_PTRS = 1
   integer(c_int), parameter :: OAKG_EIG_JACOBI = 0, OAKG_EIG_QL = 1
+  ! launch shapes of the zone loop (oakg_set_tuning)
+  integer(c_int), parameter :: OAKG_TUNE_SLOTS = 1, OAKG_TUNE_QL_ZONES = 2, OAKG_TUNE_GRAPH = 3, &
+       OAKG_TUNE_ORDER = 6, OAKG_TUNE_TRIDIAG_SPLIT = 7, OAKG_TUNE_DUAL = 8
 
   type, bind(C) :: oakg_zones
     integer(c_int64_t) :: nzones
@@ -59,6 +62,12 @@ module oakg
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: method
+    end function
+
+    integer(c_int) function oakg_set_tuning(ctx, key, value) bind(C, name="oakg_set_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: key, value
     end function
 
     integer(c_int) function oakg_local_analysis(ctx, zones, obs, N, xf, Hxf, Sf, ldSf, HSf, ldHSf, &

@@ -148,25 +148,36 @@ def innovation_nll(log_sigma, log_L, log_infl, prob, ctx=None):
     return _fn.apply(log_sigma, log_L, log_infl, prob, ctx or default_context())
 
 
-def tune(prob, log_sigma0, L0, infl0, steps=50, lr=0.05, ctx=None, callback=None):
+def tune(prob, log_sigma0, L0, infl0, steps=50, lr=0.05, ctx=None, callback=None,
+         classes=None, fix_L=False):
     """Adam on (log sigma, log L, log inflation) minimising J; returns the
-    tuned values and the history of J."""
+    tuned per-observation log sigma, L, the inflation and the history of J.
+
+    classes: optional integer class of each observation; sigma is then one
+    parameter per class (log_sigma0 per class), which is what a single
+    innovation vector can identify: a free sigma per observation fits each
+    residual.  fix_L keeps the localisation length at L0."""
     import torch
     dev = prob.yo.device
     ls = torch.nn.Parameter(torch.as_tensor(log_sigma0, dtype=torch.float64, device=dev).clone())
     lL = torch.nn.Parameter(torch.tensor(float(np.log(L0)), dtype=torch.float64, device=dev))
     la = torch.nn.Parameter(torch.tensor(float(np.log(infl0)), dtype=torch.float64, device=dev))
-    opt = torch.optim.Adam([ls, lL, la], lr=lr)
+    cls = None
+    if classes is not None:
+        cls = torch.as_tensor(np.asarray(classes), dtype=torch.int64, device=dev)
+    expand = (lambda p: p[cls]) if cls is not None else (lambda p: p)
+    opt = torch.optim.Adam([ls, la] if fix_L else [ls, lL, la], lr=lr)
     hist = []
     for it in range(steps):
         opt.zero_grad()
-        J = innovation_nll(ls, lL, la, prob, ctx)
+        J = innovation_nll(expand(ls), lL, la, prob, ctx)
         J.backward()
         hist.append(float(J.detach()))
         if callback:
             callback(it, float(J.detach()), ls, lL, la)
         opt.step()
     with torch.no_grad():
-        J = innovation_nll(ls, lL, la, prob, ctx)
-    hist.append(float(J))
-    return ls.detach(), float(torch.exp(lL.detach())), float(torch.exp(la.detach())), hist
+        J = innovation_nll(expand(ls), lL, la, prob, ctx)
+        hist.append(float(J))
+        out = expand(ls).detach()
+    return out, float(torch.exp(lL.detach())), float(torch.exp(la.detach())), hist

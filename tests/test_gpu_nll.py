@@ -88,3 +88,33 @@ def test_autograd_function_and_adam_decrease(small):
     assert torch.isfinite(ls.grad).all() and torch.isfinite(lL.grad) and torch.isfinite(la.grad)
     _, L1, a1, hist = tune(dprob, ls.detach(), 3.0, 1.0, steps=10, lr=0.05)
     assert hist[-1] < hist[0]
+
+
+def test_tuning_recovers_generating_sigma_and_inflation():
+    """One zone holding all m observations (w = 1), innovations drawn from the
+    model itself, d ~ N(0, a^2 S S^T + sigma^2 I): Adam on a tied log sigma
+    and the log inflation (L fixed) finds the generating values, sigma to the
+    accuracy m = 2000 residuals give (a few %), the inflation to that of N = 32
+    directions."""
+    import torch
+    from oak_amd.assimilation import Geometry
+    from oak_amd.nll import NLLProblem, tune
+    g = np.random.default_rng(41)
+    m, N, sig, a_true = 2000, 32, 0.5, 1.3
+    S = g.standard_normal((m, N)) / np.sqrt(N - 1)
+    d = a_true * S @ g.standard_normal(N) + sig * g.standard_normal(m)
+    Hxf = g.standard_normal(m)
+    dev = torch.device("cuda:0")
+    t = lambda x, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(x)).to(dev, dt)
+    ox, oy = g.uniform(-1, 1, m), g.uniform(-1, 1, m)
+    geo = Geometry(t(np.array([0, 1]), torch.int64), t(np.zeros((1, 2))), t(np.array([1e6])),
+                   t(np.array([1e9])), t(ox), t(oy))
+    prob = NLLProblem(geo, t(Hxf), t(Hxf + d), t(S.T).t(), N)
+    ls, L1, a1, hist = tune(prob, torch.tensor([float(np.log(1.5))], dtype=torch.float64),
+                            1e6, 0.6, steps=300, lr=0.05, classes=np.zeros(m, np.int64),
+                            fix_L=True)
+    s1 = float(torch.exp(ls[0]))
+    assert hist[-1] < hist[0]
+    assert abs(s1 / sig - 1.0) < 0.08, s1
+    assert abs(a1 / a_true - 1.0) < 0.5, a1
+    assert L1 == pytest.approx(1e6)

@@ -345,27 +345,26 @@ def roofline(F, phase, mixed, eig_name, config, N):
             "unit": "TFLOP/s", "frac": ach / peak, "traffic": phase_traffic(kern, mixed, config)}
 
 
-PHASE_KERNELS = {"eig": ["k_tridiag", "k_formq128", "k_tql", "k_tql_apply128"],
-                 "gram": ["k_gram128"], "update": ["k_xform128"]}
+# the kernels of each phase at N > 64 (the observation-space bucket's
+# kernels and the Jacobi fallback passes included)
+PHASE_KERNELS = {"eig": ["k_tridiag", "k_formq128", "k_tql", "k_tql_apply128", "k_eig128",
+                         "k_tql_apply", "k_eigvec_epi", "k_eig", "k_dual_lift"],
+                 "gram": ["k_gram128", "k_dual_gram"], "update": ["k_xform128", "k_xform128f"]}
 
 
 def phase_traffic(kern, mixed, config):
-    """HBM bytes moved by the phase's kernels in one analysis, from the
-    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs committed under
-    profiles/ (config 2 only; None otherwise)."""
+    """HBM bytes moved by the phase's kernels in one analysis of the given
+    precision, from the calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    committed under profiles/ (per call, every dispatch of a kernel in the
+    call summed; config 2 only, None otherwise)."""
     path = os.path.join(ROOT, "profiles", "traffic_config%d.json" % config)
     if not os.path.exists(path):
         return None
-    tab = json.load(open(path))
+    tab = json.load(open(path)).get("bytes_per_call", {}).get("mixed" if mixed else "fp64")
     names = PHASE_KERNELS.get(kern)
-    if kern == "update" and mixed:
-        names = ["k_xform128f"]
-    if not names:
+    if not tab or not names:
         return None
-    per = tab["bytes_per_dispatch"]
-    if not all(k in per for k in names):
-        return None
-    return float(sum(per[k]["read"] + per[k]["write"] for k in names) * tab["chunks_per_call"])
+    return float(sum(tab[k]["read"] + tab[k]["write"] for k in names if k in tab))
 
 
 def run_config3(args, device):
@@ -496,8 +495,10 @@ def main():
     g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
     print("# generated %s in %.1f s" % (desc, time.time() - t0), file=sys.stderr, flush=True)
     n, N = DP.E.shape
-    eig_name = (("k_tridiag+k_formq128+k_tql+k_tql_apply128 (Householder, Q in blocked WY form "
-                 "on FP64 MFMA, implicit QL, eigenvector epilogue; FP64)" if N > 64 else
+    eig_name = (("k_tridiag (two launches)+k_formq128+k_tql+k_tql_apply128 (Householder, Q in "
+                 "blocked WY form on FP64 MFMA, implicit QL, eigenvector epilogue; FP64), with "
+                 "the zones of p < 64 through k_tridiag+k_tql+k_tql_apply+k_eigvec_epi at 64 "
+                 "and k_dual_lift" if N > 64 else
                  "k_tridiag+k_tql+k_tql_apply+k_eigvec_epi (Householder + implicit QL, FP64)")
                 if args.eig == "ql" else "k_eig (one-sided Jacobi, FP64)")
     eig_id = OAKG_EIG_QL if args.eig == "ql" else OAKG_EIG_JACOBI

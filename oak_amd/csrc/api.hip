@@ -460,7 +460,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   const bool ql = c->eig_method == 1;
   // zones with 1 <= p < dual_lt in observation space (dual.hip): N > 64, the
   // analysis modes (not the likelihood), not in a no-allocation call
-  const int dual_lt = (NP == 128 && base.mode != 2 && !c->arena) ? c->tune_dual : 0;
+  const int dual_lt = (NP == 128 && base.mode != 2 && !c->arena && nz > 0) ? c->tune_dual : 0;
   constexpr int PD = 64;
   const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0) +

@@ -125,6 +125,35 @@ def test_dual_locanalysis_with_amplitudes():
         ref_ctx.close(); ctx.close()
 
 
+def test_dual_excluded_observations_and_single_observation_zones():
+    """Excluded observations (rho = 0: zero rows of S~) inside dual zones, zones
+    whose observations are all excluded (G = 0, mu = 1: a = 0, T = Omega) and
+    zones with one observation, against the oracle with the same exclusions."""
+    from oak_amd import assimilation as A
+    P = _problem(128, 700)
+    H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
+    g = np.random.default_rng(3)
+    ex = (g.uniform(size=P.m) < 0.3).astype(np.uint8)
+    geo = dict(zone_ptr=P.zone_ptr, zpos=P.zpos, L=P.corr_len, cutoff=P.max_len, metric=0,
+               loctype=1, ox=P.ox, oy=P.oy)
+    Eo, _, cnt = O.ensemble_analysis(geo, (P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
+                                     dex=ex.astype(float))
+    assert ((cnt >= 1) & (cnt < 64)).sum() >= 20
+    ctx = _ctx(dual=64)
+    Ea = A.ensemble_analysis(_geo(P), H, P.Hshift, P.yo, P.rmse2, P.E, excluded=ex, ctx=ctx)
+    xf = P.E.mean(axis=1)
+    assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64
+    # every observation of the problem excluded: each zone keeps its forecast
+    # mean, and its anomalies are the forecast's rotated by Omega
+    exa = np.ones(P.m, np.uint8)
+    Eo2, _, _ = O.ensemble_analysis(geo, (P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
+                                    dex=exa.astype(float))
+    Ea2 = A.ensemble_analysis(_geo(P), H, P.Hshift, P.yo, P.rmse2, P.E, excluded=exa, ctx=ctx)
+    assert rel_err(Ea2 - xf[:, None], Eo2 - xf[:, None]) < TOL64
+    assert np.abs(Ea2.mean(axis=1) - xf).max() < 1e-12
+    ctx.close()
+
+
 @pytest.mark.parametrize("N", [100, 128])
 def test_three_launch_reduction_vs_oracle(N):
     """OAKG_TUNE_TRIDIAG_SPLIT = 2 (steps 32..63 of the Householder reduction

@@ -136,8 +136,9 @@ def test_dual_excluded_observations_and_single_observation_zones():
     ex = (g.uniform(size=P.m) < 0.3).astype(np.uint8)
     geo = dict(zone_ptr=P.zone_ptr, zpos=P.zpos, L=P.corr_len, cutoff=P.max_len, metric=0,
                loctype=1, ox=P.ox, oy=P.oy)
+    # the oracle takes OAK's mask D (1 = used, 0 = excluded, assimilation.F90:3086-3092)
     Eo, _, cnt = O.ensemble_analysis(geo, (P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
-                                     dex=ex.astype(float))
+                                     dex=1.0 - ex)
     assert ((cnt >= 1) & (cnt < 64)).sum() >= 20
     ctx = _ctx(dual=64)
     Ea = A.ensemble_analysis(_geo(P), H, P.Hshift, P.yo, P.rmse2, P.E, excluded=ex, ctx=ctx)
@@ -147,7 +148,7 @@ def test_dual_excluded_observations_and_single_observation_zones():
     # mean, and its anomalies are the forecast's rotated by Omega
     exa = np.ones(P.m, np.uint8)
     Eo2, _, _ = O.ensemble_analysis(geo, (P.Hi, P.Hj, P.Hs), P.Hshift, P.yo, P.rmse2, P.E,
-                                    dex=exa.astype(float))
+                                    dex=1.0 - exa)
     Ea2 = A.ensemble_analysis(_geo(P), H, P.Hshift, P.yo, P.rmse2, P.E, excluded=exa, ctx=ctx)
     assert rel_err(Ea2 - xf[:, None], Eo2 - xf[:, None]) < TOL64
     assert np.abs(Ea2.mean(axis=1) - xf).max() < 1e-12

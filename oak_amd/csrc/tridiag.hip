@@ -542,7 +542,10 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
 // written with its columns reversed, as k_tridiag writes it for NP < 128.
 // ---------------------------------------------------------------------------
 struct FormQLds {
-  static constexpr int LV = 17;  // padded row of Vb and Tb
+  // padded row of Vb and Tb: with 18 doubles the MFMA operand reads of 64
+  // lanes (16 rows x 4, or 16 columns x 4) and the staging writes below fall
+  // on distinct bank pairs as far as 64 lanes allow
+  static constexpr int LV = 18;
   double vb[2][128 * LV];        // Vb[r][j]: row r of reflector k0 + j
   double tb[2][16 * LV];         // Tb
   double g[16 * LV];             // Vb^T Vb
@@ -569,19 +572,17 @@ __global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int32_t* 
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         q[jj][I][r] = (I == (jj ? J1 : J0) && lr + 4 * r == lc) ? 1.0 : 0.0;
-  // thread t moves reflector pj = t / 16 of a block, rows pr .. pr + 7
-  const int pj = t >> 4, pr = 8 * (t & 15);
+  // thread t moves reflector pj = t / 16 of a block, rows pr + 16 e (e < 8):
+  // 16 threads read 16 consecutive values of a reflector row, and their
+  // LDS stores (row stride LV) spread over the banks
+  const int pj = t >> 4, pr = t & 15;
   double pf[8];
   auto load_block = [&](int b) {
     const int k = 16 * b + pj;
     if (k < NP - 2) {
-      const double2* src = reinterpret_cast<const double2*>(V + (size_t)k * NP + pr);
+      const double* src = V + (size_t)k * NP + pr;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const double2 x = src[e];
-        pf[2 * e] = x.x;
-        pf[2 * e + 1] = x.y;
-      }
+      for (int e = 0; e < 8; ++e) pf[e] = src[16 * e];
     } else {
 #pragma unroll
       for (int e = 0; e < 8; ++e) pf[e] = 0.0;
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int32_t* 
       const int k = 16 * b + pj;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const int r = pr + e;
+        const int r = pr + 16 * e;
         vb[r * LV + pj] = r == k ? 0.0 : pf[e];
         if (r == k) S.tau[buf][pj] = pf[e];
       }

@@ -226,7 +226,8 @@ def run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E, Ea, label):
         # time every launch shape on this workload; the table entry is kept
         # in oak_amd/tuning.json
         run = lambda: ensemble_analysis(g, H, Hshift, yo, rmse2, E, ctx=ctx, out=Ea)
-        best, times = autotune.tune(run, ctx, log=lambda s: print(s, file=sys.stderr, flush=True))
+        best, times = autotune.tune(run, ctx, space=autotune.space_for(N),
+                                    log=lambda s: print(s, file=sys.stderr, flush=True))
         autotune.save_entry(N, ctx.precision, best, times, "config%d" % args.config)
         print("# %s autotuned: %s" % (label, best), file=sys.stderr, flush=True)
     settings = autotune.apply(ctx, N, ctx.precision)

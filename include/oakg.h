@@ -174,6 +174,12 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
  *                         G = S~ S~^T instead of M = I + S~^T S~, which have
  *                         the same nonzero spectrum.  The analysis is the
  *                         same to rounding (not bit for bit).
+ *   OAKG_TUNE_PERSIST     at N > 64, the Gram kernel as a persistent grid
+ *                         (two workgroups per CU) taking zones from work
+ *                         counters in the zone loop's order: 2 a zone per
+ *                         workgroup, 1 each wave the next zone of its
+ *                         quarter of the tiles; 0: a workgroup per zone
+ *                         (default)
  * oakg_get_tuning also reads OAKG_TUNE_GRAPH_REPLAYS / _CAPTURES (counts). */
 #define OAKG_TUNE_SLOTS 1
 #define OAKG_TUNE_QL_ZONES 2
@@ -183,6 +189,7 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
 #define OAKG_TUNE_ORDER 6
 #define OAKG_TUNE_TRIDIAG_SPLIT 7
 #define OAKG_TUNE_DUAL 8
+#define OAKG_TUNE_PERSIST 9
 int oakg_set_tuning(oakg_ctx* ctx, int key, int value);
 int oakg_get_tuning(oakg_ctx* ctx, int key, int64_t* value);
 /* Per-zone local observation counts p_z of the last call (host array). */

@@ -31,7 +31,7 @@ EXPORTS = [
 ]
 OAKG_TUNE_SLOTS, OAKG_TUNE_QL_ZONES, OAKG_TUNE_GRAPH = 1, 2, 3
 OAKG_TUNE_GRAPH_REPLAYS, OAKG_TUNE_GRAPH_CAPTURES, OAKG_TUNE_ORDER = 4, 5, 6
-OAKG_TUNE_TRIDIAG_SPLIT, OAKG_TUNE_DUAL = 7, 8
+OAKG_TUNE_TRIDIAG_SPLIT, OAKG_TUNE_DUAL, OAKG_TUNE_PERSIST = 7, 8, 9
 
 
 class OakError(RuntimeError):
@@ -195,15 +195,17 @@ class Context:
         self.check(load().oakg_set_eigensolver(self._h, int(method)))
 
     def set_tuning(self, slots=None, ql_zones=None, graph=None, order=None, tri_split=None,
-                   dual=None):
+                   dual=None, persist=None):
         """Zone chunks in flight, zones per QL wave, HIP-graph replay of the
         zone loop, zones ordered by observation count, the N > 64 Householder
         reduction as two kernels, zones with fewer than `dual` observations
-        solved in observation space (oakg_set_tuning)."""
+        solved in observation space, the N > 64 Gram as a persistent kernel
+        (oakg_set_tuning)."""
         L = load()
         for key, v in ((OAKG_TUNE_SLOTS, slots), (OAKG_TUNE_QL_ZONES, ql_zones),
                        (OAKG_TUNE_GRAPH, graph), (OAKG_TUNE_ORDER, order),
-                       (OAKG_TUNE_TRIDIAG_SPLIT, tri_split), (OAKG_TUNE_DUAL, dual)):
+                       (OAKG_TUNE_TRIDIAG_SPLIT, tri_split), (OAKG_TUNE_DUAL, dual),
+                       (OAKG_TUNE_PERSIST, persist)):
             if v is not None:
                 self.check(L.oakg_set_tuning(self._h, key, int(v)))
 
@@ -213,6 +215,7 @@ class Context:
         for name, key in (("slots", OAKG_TUNE_SLOTS), ("ql_zones", OAKG_TUNE_QL_ZONES),
                           ("graph", OAKG_TUNE_GRAPH), ("order", OAKG_TUNE_ORDER),
                           ("tri_split", OAKG_TUNE_TRIDIAG_SPLIT), ("dual", OAKG_TUNE_DUAL),
+                          ("persist", OAKG_TUNE_PERSIST),
                           ("graph_replays", OAKG_TUNE_GRAPH_REPLAYS),
                           ("graph_captures", OAKG_TUNE_GRAPH_CAPTURES)):
             v = C.c_int64(0)

@@ -10,9 +10,12 @@ The knobs (oakg_set_tuning) change how the zones are scheduled:
              with fewer registers and more zones per CU) or three (the
              middle one on 6 x 6 register tiles),
   dual       the bucket bound below which a zone is solved in its p x p
-             observation space.
-All but dual and tri_split = 2 give the same bits; those give the same
-analysis to rounding.  ``tune`` times every candidate on the caller's own workload
+             observation space,
+  persist    the Gram kernel as a persistent grid taking zones from a work
+             counter (2: a zone per workgroup).
+tri_split, dual and persist act at N > 64 only (``space_for`` leaves them
+out below).  All but dual and tri_split = 2 give the same bits; those give
+the same analysis to rounding.  ``tune`` times every candidate on the caller's own workload
 (device-resident inputs, the same call repeated), keeps the fastest, and
 returns a table entry; ``apply`` sets a context from a table (a JSON file
 keyed by ensemble size and precision, with the workload it was tuned on)."""
@@ -22,14 +25,23 @@ import os
 import time
 
 CANDIDATES = {"slots": (2, 3, 4), "ql_zones": (4, 8, 16), "graph": (0, 1), "order": (0, 1),
-              "tri_split": (0, 1, 2), "dual": (0, 32, 64)}
-DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0, "order": 0, "tri_split": 0, "dual": 0}
+              "tri_split": (0, 1, 2), "dual": (0, 32, 64), "persist": (0, 2)}
+DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0, "order": 0, "tri_split": 0, "dual": 0,
+           "persist": 0}
+WIDE_ONLY = ("tri_split", "dual", "persist")  # knobs of the N > 64 kernels
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
 def key(N, precision):
     return "N%d_%s" % (int(N), "mixed" if int(precision) == 1 else "fp64")
+
+
+def space_for(N):
+    """The candidates that change anything at ensemble size N."""
+    if int(N) > 64:
+        return dict(CANDIDATES)
+    return {k: v for k, v in CANDIDATES.items() if k not in WIDE_ONLY}
 
 
 def tune(run, ctx, repeats=2, space=None, log=None):
@@ -77,9 +89,10 @@ def apply(ctx, N, precision, path=TABLE):
 
 
 def save_entry(N, precision, settings, times, workload, path=TABLE):
+    """Stores the entry of (N, precision); its timings fastest first."""
     tab = load_table(path)
-    tab[key(N, precision)] = {"settings": settings, "workload": workload,
+    tab[key(N, precision)] = {"settings": dict(sorted(settings.items())), "workload": workload,
                               "ms": {k: round(v, 3) for k, v in sorted(times.items(),
                                                                        key=lambda kv: kv[1])}}
     with open(path, "w") as f:
-        json.dump(tab, f, indent=1, sort_keys=True)
+        json.dump(dict(sorted(tab.items())), f, indent=1)

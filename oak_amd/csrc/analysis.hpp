@@ -95,6 +95,10 @@ struct ChunkArgs {
   int eig_method;
   int ql_zw;                // zones per wave of the QL kernel (4, 8, 16 or 32)
   int tri_split;            // NP = 128: the Householder reduction as two launches
+  unsigned* persist_wq;     // NP = 128: work counter of the persistent Gram (NULL: one
+                            // workgroup per zone)
+  int persist_item;         // its scheduling: 1 a zone per workgroup, 0 each wave its quarter
+  int persist_grid;         // its workgroups (persist_grid128)
   double *dvec, *evec;      // tridiagonal / eigenvalues, NP x nzc
   void* rlog;               // QL rotation log, rlog_cap records of 16 B per zone
   int64_t rlog_cap;
@@ -149,6 +153,9 @@ void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 // NP = 128 in two parts (Gram + eigensolve, then transform + update), so
 // that the zones solved in observation space can run in between
+// Workgroups of the persistent Gram that fit on the current device at once
+// (0 when the runtime cannot say).
+int persist_grid128();
 void launch_solve128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_apply128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 // eigensolver of a chunk for NP <= 64 (QL pipeline or Jacobi, with fallback)

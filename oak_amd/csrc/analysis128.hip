@@ -14,6 +14,8 @@
 //              run from it (FP64 MFMA; FP32 MFMA for FP32 members)
 #include <type_traits>
 
+#include <algorithm>
+
 #include "analysis.hpp"
 #include "common.hpp"
 
@@ -147,7 +149,94 @@ __global__ __launch_bounds__(256) void k_gram128(int64_t z0, const int32_t* __re
   }
 }
 
-// ---------------------------------------------------------------------------
+// The same Gram as a persistent kernel: a fixed grid takes zones from work
+// counters in the order of the zone loop until the chunk is done.  Wave w of
+// a workgroup always computes quarter w of a zone's tiles (the quarters need
+// no common barrier), so the quarter is fixed per wave and its tile loop is
+// compiled alone.  WG: the workgroup takes one zone for its four waves from
+// counter 0 (the zone broadcast through LDS); else each wave takes the next
+// zone of its own quarter from counter w.  The counters are cleared by the
+// caller (a fill kernel) before each launch; every wave leaves its loop once
+// the zone it reads passes the chunk (with WG all four waves read the same
+// value, so they pass the same number of barriers).
+constexpr int kPersistBlocksPerCU = 2;  // the register bound the kernel is compiled for
+
+__global__ void k_clear_wq(unsigned* wq) {
+  if (threadIdx.x < 4) wq[threadIdx.x] = 0u;
+}
+
+template <int W, typename TS, bool WG>
+__device__ __forceinline__ void gram128_persist_loop(int64_t z0, const int32_t* __restrict__ zorder,
+                                                     int nzc, unsigned* __restrict__ counter,
+                                                     unsigned* sh_item,
+                                                     const int64_t* __restrict__ sel_ptr,
+                                                     const int32_t* __restrict__ sel_idx,
+                                                     const double* __restrict__ sel_w, int64_t m,
+                                                     const double* __restrict__ rho,
+                                                     const double* __restrict__ innov,
+                                                     const TS* __restrict__ HS,
+                                                     double* __restrict__ mat,
+                                                     double* __restrict__ vec,
+                                                     unsigned* __restrict__ err, int lane) {
+  auto zone = [&](unsigned item) {
+    const int64_t z = zone_at(zorder, z0 + (int64_t)item);
+    if (z < 0) return;  // the zone is solved by the other pipeline
+    const int64_t p0 = sel_ptr[z];
+    const int64_t p = sel_ptr[z + 1] - p0;
+    double* M = mat + (size_t)item * P128 * P128;
+    double* bout = vec + (size_t)item * 4 * P128;
+    if (p == 0) {  // identity: rows 32 W .. 32 W + 31; b from quarter 0
+      for (int e = lane; e < 32 * P128; e += 64) {
+        const int r = 32 * W + e / P128, c = e % P128;
+        M[r * P128 + c] = r == c ? 1.0 : 0.0;
+      }
+      if (W == 0)
+        for (int e = lane; e < P128; e += 64) bout[e] = 0.0;
+      return;
+    }
+    gram128_wave<W, TS>(p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bout, err, lane);
+  };
+  if constexpr (WG) {
+    // wave 0 takes the zone; the first barrier publishes it (the store is
+    // waited for before the barrier), the second keeps the slot until every
+    // wave has read it
+    for (;;) {
+      if (W == 0 && lane == 0) *sh_item = atomicAdd(counter, 1u);
+      __syncthreads();
+      const unsigned item = *sh_item;
+      __syncthreads();
+      if (item >= (unsigned)nzc) break;  // the same value in every wave
+      zone(item);
+    }
+  } else {
+    for (;;) {
+      unsigned item = 0;
+      if (lane == 0) item = atomicAdd(counter + W, 1u);
+      item = __builtin_amdgcn_readfirstlane(item);
+      if (item >= (unsigned)nzc) break;
+      zone(item);
+    }
+  }
+}
+
+template <typename TS, bool WG>
+__global__ __launch_bounds__(256, kPersistBlocksPerCU) void k_gram128_persist(
+    int64_t z0, const int32_t* __restrict__ zorder, int nzc, unsigned* __restrict__ counter,
+    const int64_t* __restrict__ sel_ptr, const int32_t* __restrict__ sel_idx,
+    const double* __restrict__ sel_w, int64_t m, const double* __restrict__ rho,
+    const double* __restrict__ innov, const TS* __restrict__ HS, double* __restrict__ mat,
+    double* __restrict__ vec, unsigned* __restrict__ err) {
+  __shared__ unsigned sh_item;
+  // the wave index as a scalar: each wave runs one branch, with no exec mask
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  switch (wave) {
+    case 0: gram128_persist_loop<0, TS, WG>(z0, zorder, nzc, counter, &sh_item, sel_ptr, sel_idx, sel_w, m, rho, innov, HS, mat, vec, err, lane); break;
+    case 1: gram128_persist_loop<1, TS, WG>(z0, zorder, nzc, counter, &sh_item, sel_ptr, sel_idx, sel_w, m, rho, innov, HS, mat, vec, err, lane); break;
+    case 2: gram128_persist_loop<2, TS, WG>(z0, zorder, nzc, counter, &sh_item, sel_ptr, sel_idx, sel_w, m, rho, innov, HS, mat, vec, err, lane); break;
+    default: gram128_persist_loop<3, TS, WG>(z0, zorder, nzc, counter, &sh_item, sel_ptr, sel_idx, sel_w, m, rho, innov, HS, mat, vec, err, lane); break;
+  }
+}
+
 // Block one-sided Jacobi, NP = 128
 // ---------------------------------------------------------------------------
 struct Eig128Lds {
@@ -951,9 +1040,39 @@ __global__ __launch_bounds__(256) void k_xform128f(int64_t z0, const int32_t* __
   }
 }
 
+int persist_grid128() {
+  // the occupancy the launch bounds guarantee (the runtime's occupancy
+  // query reports one workgroup per CU for these 256-VGPR waves)
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return kPersistBlocksPerCU * ncu;
+}
+
 void launch_solve128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_GRAM);
-  if (A.prec == 1)
+  if (A.persist_wq) {
+    hipLaunchKernelGGL(k_clear_wq, dim3(1), dim3(64), 0, st, A.persist_wq);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(A.persist_grid, A.nzc));
+    const dim3 g(grid), b(256);
+    if (A.prec == 1 && A.persist_item == 1)
+      hipLaunchKernelGGL((k_gram128_persist<float, true>), g, b, 0, st, A.z0, A.zorder, A.nzc,
+                         A.persist_wq, A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32,
+                         A.mat, A.vec, A.err);
+    else if (A.prec == 1)
+      hipLaunchKernelGGL((k_gram128_persist<float, false>), g, b, 0, st, A.z0, A.zorder, A.nzc,
+                         A.persist_wq, A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32,
+                         A.mat, A.vec, A.err);
+    else if (A.persist_item == 1)
+      hipLaunchKernelGGL((k_gram128_persist<double, true>), g, b, 0, st, A.z0, A.zorder, A.nzc,
+                         A.persist_wq, A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS,
+                         A.mat, A.vec, A.err);
+    else
+      hipLaunchKernelGGL((k_gram128_persist<double, false>), g, b, 0, st, A.z0, A.zorder, A.nzc,
+                         A.persist_wq, A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS,
+                         A.mat, A.vec, A.err);
+  } else if (A.prec == 1)
     hipLaunchKernelGGL(k_gram128<float>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec, A.err);
   else

@@ -70,6 +70,7 @@ struct oakg_ctx {
   int tune_order = 0;  // 1: zones ordered by observation count (k_zone_order)
   int tune_tri_split = 0;  // 1: N = 128 Householder reduction as two launches
   int tune_dual = 0;  // N = 128: zones with 1 <= p < tune_dual solved in observation space
+  int tune_persist = 0;  // N = 128: the Gram as a persistent kernel (2 zone per workgroup, 1 per wave)
   // the zone loop captured in a HIP graph and replayed while its launches
   // (kernel arguments, buffers, streams) stay the same
   bool graph = false;
@@ -519,6 +520,16 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     vec_d = (double*)wsget(c, Workspace::VEC_D, sizeof(double) * (size_t)4 * PD * zs);
     if (!zord_p || !zord_d || !mat_d || !ymat_d || !vec_d) check(c, hipErrorOutOfMemory, "workspace");
   }
+  // work counters of the persistent Gram, one per slot on its own 256 B
+  const int persist_item = c->tune_persist == 2 ? 1 : 0;
+  const int persist_grid =
+      NP == 128 && c->tune_persist != 0 && !c->arena && nz > 0 ? persist_grid128() : 0;
+  const bool persist = persist_grid > 0;
+  unsigned* wq = nullptr;
+  if (persist) {
+    wq = (unsigned*)wsget(c, Workspace::PERSIST_WQ, (size_t)256 * OAKG_PIPE_SLOTS);
+    if (!wq) check(c, hipErrorOutOfMemory, "workspace");
+  }
   // streams and events of the pipeline, created before any capture
   if (nslot > 1) {
     if (!c->ev_fork) check(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
@@ -538,8 +549,8 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     struct Key {
       ChunkArgs base;
       int64_t nz, zc, cap;
-      int NP, nslot, ql, ql_zw, order, tri_split, dual_lt;
-      const void* buf[14];
+      int NP, nslot, ql, ql_zw, order, tri_split, dual_lt, persist;
+      const void* buf[15];
       hipStream_t stream;
     } kk;
     memset(&kk, 0, sizeof kk);
@@ -554,8 +565,9 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     kk.order = order;
     kk.tri_split = c->tune_tri_split;
     kk.dual_lt = dual_lt;
-    const void* bufs[14] = {mat,  ymat,   vec,    dvec,   evec,  rlog,  tst,
-                            sweeps_all, zorder, zord_p, zord_d, mat_d, ymat_d, vec_d};
+    kk.persist = persist ? 1 + persist_item : 0;
+    const void* bufs[15] = {mat,  ymat,   vec,    dvec,   evec,  rlog,  tst,   sweeps_all,
+                            zorder, zord_p, zord_d, mat_d, ymat_d, vec_d, wq};
     memcpy(kk.buf, bufs, sizeof bufs);
     kk.stream = c->stream;
     key.assign((const char*)&kk, (const char*)&kk + sizeof kk);
@@ -628,6 +640,9 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     A.tql_status = ql ? tst + zo : nullptr;
     A.ql_zw = c->tune_ql_zw;
     A.tri_split = c->tune_tri_split;
+    A.persist_wq = persist ? wq + (size_t)64 * sl : nullptr;
+    A.persist_item = persist_item;
+    A.persist_grid = persist_grid;
     PhaseTimer* tm = c->timing ? &c->timer : nullptr;
     if (dual_lt) {
       // ensemble-space zones, then observation-space zones (writing their
@@ -1652,6 +1667,10 @@ int oakg_set_tuning(oakg_ctx* c, int key, int value) {
       if (value != 0 && value != 16 && value != 32 && value != 64) break;
       c->tune_dual = value;
       return OAKG_OK;
+    case OAKG_TUNE_PERSIST:
+      if (value < 0 || value > 2) break;
+      c->tune_persist = value;
+      return OAKG_OK;
     case OAKG_TUNE_GRAPH:
       if (value != 0 && value != 1) break;
       c->graph = value != 0;
@@ -1677,6 +1696,7 @@ int oakg_get_tuning(oakg_ctx* c, int key, int64_t* value) {
     case OAKG_TUNE_ORDER: *value = c->tune_order; return OAKG_OK;
     case OAKG_TUNE_TRIDIAG_SPLIT: *value = c->tune_tri_split; return OAKG_OK;
     case OAKG_TUNE_DUAL: *value = c->tune_dual; return OAKG_OK;
+    case OAKG_TUNE_PERSIST: *value = c->tune_persist; return OAKG_OK;
     case OAKG_TUNE_GRAPH_REPLAYS: *value = c->graph_replays; return OAKG_OK;
     case OAKG_TUNE_GRAPH_CAPTURES: *value = c->graph_captures; return OAKG_OK;
     default: break;

@@ -14,7 +14,7 @@ module oakg
   integer(c_int), parameter :: OAKG_EIG_JACOBI = 0, OAKG_EIG_QL = 1
   ! launch shapes of the zone loop (oakg_set_tuning)
   integer(c_int), parameter :: OAKG_TUNE_SLOTS = 1, OAKG_TUNE_QL_ZONES = 2, OAKG_TUNE_GRAPH = 3, &
-       OAKG_TUNE_ORDER = 6, OAKG_TUNE_TRIDIAG_SPLIT = 7, OAKG_TUNE_DUAL = 8
+       OAKG_TUNE_ORDER = 6, OAKG_TUNE_TRIDIAG_SPLIT = 7, OAKG_TUNE_DUAL = 8, OAKG_TUNE_PERSIST = 9
 
   type, bind(C) :: oakg_zones
     integer(c_int64_t) :: nzones

@@ -689,9 +689,18 @@ __device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) 
 // Implicit QL with Wilkinson shifts on (d, e), one lane per zone (EISPACK
 // tql2 ordering: eigenvalue l converges at the top of the block l..m).  The
 // end m of the unreduced block is carried from one sweep to the next: the
-// negligibility test of e_j is made as soon as e_j, d_j and d_{j+1} are final
-// in the sweep, so a full rescan is only needed after a 1 x 1 deflation.
-template <int NP, int ZW>
+// negligibility test of e_j is made as soon as e_j is final in the sweep, so
+// a full rescan is only needed after a 1 x 1 deflation.  ABS = false: e_j is
+// negligible when |e_j| <= eps (|d_j| + |d_{j+1}|), relative to its
+// neighbours as in LAPACK's dsteqr (the FP64 path).  ABS = true (the mixed
+// path, whose error bound is 1e-4): |e_j| <= eps ||T|| with
+// ||T|| = max_k (|d_k| + |e_k|), EISPACK tql2's absolute test.  T comes from a
+// Householder reduction whose backward error is already of order eps ||M||,
+// and every matrix solved here is I plus a positive semidefinite one, so the
+// absolute test keeps the solve backward stable; it saves the iterations on
+// rounding-level e_j next to eigenvalues near 1 (the N - p unit eigenvalues
+// of a zone with p < N observations most of all).
+template <int NP, int ZW, bool ABS>
 __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                             double* __restrict__ dvec, double* __restrict__ evec,
                                             double2* __restrict__ rlog, int64_t cap,
@@ -720,8 +729,12 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
   double2* lg = rlog + (valid ? rlog_off(zl, cap) : 0);
   const double eps = 2.220446049250313e-16;
   if (valid) {
+    double tn = fabs(d[NP - 1][lane]);
+    if (ABS)
+      for (int k = 0; k < NP - 1; ++k) tn = fmax(tn, fabs(d[k][lane]) + fabs(e[k][lane]));
+    const double tol = eps * tn;
     auto negl = [&](int j) {
-      return fabs(e[j][lane]) <= eps * (fabs(d[j][lane]) + fabs(d[j + 1][lane]));
+      return fabs(e[j][lane]) <= (ABS ? tol : eps * (fabs(d[j][lane]) + fabs(d[j + 1][lane])));
     };
     auto scan = [&](int from) {
       int m = from;
@@ -783,7 +796,7 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
         g = fma(c, rr, -bb);
         lg[rlog_rec(pos)] = make_double2(c, s);
         pos += under ? 0 : 1;
-        const bool small = fabs(r) <= eps * (fabs(dn) + fabs(dnext2));
+        const bool small = fabs(r) <= (ABS ? tol : eps * (fabs(dn) + fabs(dnext2)));
         jmin = (!under & (i + 1 < m) & small) ? i + 1 : jmin;  // no short circuit: no branch
         dnext2 = dn;
         ei = ein;
@@ -800,7 +813,7 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
       d[l][lane] = dl2;
       e[l][lane] = g;
       e[m][lane] = 0.0;
-      if (fabs(g) <= eps * (fabs(dl2) + fabs(dnext2))) {
+      if (fabs(g) <= (ABS ? tol : eps * (fabs(dl2) + fabs(dnext2)))) {
         ++l;
         if (l >= NP - 1) break;
         iter = 0;
@@ -1370,9 +1383,18 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
     hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr, A.ymat);
   // zones per QL wave: a tuning knob (the rotation log layout groups 64
   // zones whatever the wave width)
-#define OAK_TQL(ZW)                                                                               \
-  hipLaunchKernelGGL((k_tql<NP, ZW>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc, A.z0, A.zorder, \
-                     A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap, A.tql_status, A.err)
+  // (the deflation test: relative for FP64, absolute for the mixed path)
+#define OAK_TQL(ZW)                                                                                       \
+  do {                                                                                                    \
+    if (A.prec == 1)                                                                                      \
+      hipLaunchKernelGGL((k_tql<NP, ZW, true>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc,     \
+                         A.z0, A.zorder, A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap,         \
+                         A.tql_status, A.err);                                                            \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_tql<NP, ZW, false>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc,    \
+                         A.z0, A.zorder, A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap,         \
+                         A.tql_status, A.err);                                                            \
+  } while (0)
   if (A.ql_zw == 4)
     OAK_TQL(4);
   else if (A.ql_zw == 8)

@@ -690,16 +690,20 @@ __device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) 
 // tql2 ordering: eigenvalue l converges at the top of the block l..m).  The
 // end m of the unreduced block is carried from one sweep to the next: the
 // negligibility test of e_j is made as soon as e_j is final in the sweep, so
-// a full rescan is only needed after a 1 x 1 deflation.  ABS = false: e_j is
-// negligible when |e_j| <= eps (|d_j| + |d_{j+1}|), relative to its
-// neighbours as in LAPACK's dsteqr (the FP64 path).  ABS = true (the mixed
-// path, whose error bound is 1e-4): |e_j| <= eps ||T|| with
-// ||T|| = max_k (|d_k| + |e_k|), EISPACK tql2's absolute test.  T comes from a
-// Householder reduction whose backward error is already of order eps ||M||,
-// and every matrix solved here is I plus a positive semidefinite one, so the
-// absolute test keeps the solve backward stable; it saves the iterations on
+// a full rescan is only needed after a 1 x 1 deflation.  ABS = true (the
+// mixed path, whose error bound is 1e-4): e_j is negligible when
+// |e_j| <= tol = eps ||T|| with ||T|| = max_k (|d_k| + |e_k|), EISPACK tql2's
+// absolute test.  T comes from a Householder reduction whose backward error
+// is already of order eps ||M||, and every matrix solved here is I plus a
+// positive semidefinite one, so the absolute test keeps the solve backward
+// stable; it saves the iterations that a test relative to d_j spends on
 // rounding-level e_j next to eigenvalues near 1 (the N - p unit eigenvalues
-// of a zone with p < N observations most of all).
+// of a zone with p < N observations most of all).  ABS = false (FP64): the
+// relative test of LAPACK's dsteqr, |e_j| <= eps (|d_j| + |d_{j+1}|), except
+// between two diagonal entries within 4 tol of 1, where it is tol: there the
+// iterations only refine rounding noise inside the unit eigenvalue cluster
+// (any basis of it gives the same analysis), and the rest of the spectrum is
+// solved as dsteqr would.
 template <int NP, int ZW, bool ABS>
 __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
                                             double* __restrict__ dvec, double* __restrict__ evec,
@@ -730,12 +734,15 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
   const double eps = 2.220446049250313e-16;
   if (valid) {
     double tn = fabs(d[NP - 1][lane]);
-    if (ABS)
-      for (int k = 0; k < NP - 1; ++k) tn = fmax(tn, fabs(d[k][lane]) + fabs(e[k][lane]));
+    for (int k = 0; k < NP - 1; ++k) tn = fmax(tn, fabs(d[k][lane]) + fabs(e[k][lane]));
     const double tol = eps * tn;
-    auto negl = [&](int j) {
-      return fabs(e[j][lane]) <= (ABS ? tol : eps * (fabs(d[j][lane]) + fabs(d[j + 1][lane])));
+    // the deflation threshold of an off-diagonal entry between dj and dj1
+    auto thr = [&](double dj, double dj1) {
+      if (ABS) return tol;
+      const bool unit = fabs(dj - 1.0) <= 4.0 * tol && fabs(dj1 - 1.0) <= 4.0 * tol;
+      return unit ? tol : eps * (fabs(dj) + fabs(dj1));
     };
+    auto negl = [&](int j) { return fabs(e[j][lane]) <= thr(d[j][lane], d[j + 1][lane]); };
     auto scan = [&](int from) {
       int m = from;
       for (; m < NP - 1; ++m)
@@ -796,7 +803,7 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
         g = fma(c, rr, -bb);
         lg[rlog_rec(pos)] = make_double2(c, s);
         pos += under ? 0 : 1;
-        const bool small = fabs(r) <= (ABS ? tol : eps * (fabs(dn) + fabs(dnext2)));
+        const bool small = fabs(r) <= thr(dn, dnext2);
         jmin = (!under & (i + 1 < m) & small) ? i + 1 : jmin;  // no short circuit: no branch
         dnext2 = dn;
         ei = ein;
@@ -813,7 +820,7 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
       d[l][lane] = dl2;
       e[l][lane] = g;
       e[m][lane] = 0.0;
-      if (fabs(g) <= (ABS ? tol : eps * (fabs(dl2) + fabs(dnext2)))) {
+      if (fabs(g) <= thr(dl2, dnext2)) {
         ++l;
         if (l >= NP - 1) break;
         iter = 0;
@@ -1383,7 +1390,8 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
     hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr, A.ymat);
   // zones per QL wave: a tuning knob (the rotation log layout groups 64
   // zones whatever the wave width)
-  // (the deflation test: relative for FP64, absolute for the mixed path)
+  // (the deflation test: absolute for the mixed path, relative outside the
+  // unit eigenvalue cluster for FP64)
 #define OAK_TQL(ZW)                                                                                       \
   do {                                                                                                    \
     if (A.prec == 1)                                                                                      \

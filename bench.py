@@ -350,7 +350,8 @@ def roofline(F, phase, mixed, eig_name, config, N):
 # kernels and the Jacobi fallback passes included)
 PHASE_KERNELS = {"eig": ["k_tridiag", "k_formq128", "k_tql", "k_tql_apply128", "k_eig128",
                          "k_tql_apply", "k_eigvec_epi", "k_eig", "k_dual_lift"],
-                 "gram": ["k_gram128", "k_dual_gram"], "update": ["k_xform128", "k_xform128f"]}
+                 "gram": ["k_gram128", "k_gram128_persist", "k_dual_gram"],
+                 "update": ["k_xform128", "k_xform128f"]}
 
 
 def phase_traffic(kern, mixed, config):

@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboak_oracle.so")
+# OAK_ORACLE_LIB: another build of the same sources (the sanitised one of
+# tests/test_oracle_sanitizers.py)
+_LIB_PATH = os.environ.get("OAK_ORACLE_LIB") or os.path.join(_HERE, "liboak_oracle.so")
 _lib = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
@@ -38,6 +40,8 @@ def lib():
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
+        if os.environ.get("OAK_ORACLE_LIB"):
+            raise OracleError("OAK_ORACLE_LIB names a missing file: %s" % _LIB_PATH)
         build()
     L = C.CDLL(_LIB_PATH)
     L.oko_distance.restype = C.c_double

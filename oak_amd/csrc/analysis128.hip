@@ -45,41 +45,72 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
 #pragma unroll
   for (int J = 0; J < 8; ++J) bacc[J] = 0;
   const int64_t nsteps = (p + 3) >> 2;
-  for (int64_t s = 0; s < nsteps; ++s) {
+  // Each step's gathers hang on a load (index -> rho, innov and the row of
+  // S), so the steps are software-pipelined: the index and weight of step
+  // s + 2 and the gathers of step s + 1 are in flight while the MFMAs of step
+  // s run (same arithmetic as one step at a time).  The step is branch-free
+  // (indices past the zone's rows and invalid ones are clamped to valid
+  // entries and their values dropped), so that the compiler keeps the loads
+  // where they are issued instead of sinking them to their use.
+  struct Idx {
+    int64_t l;
+    double w;
+    bool ok;
+  };
+  bool bad = false;  // an index outside 0..m-1 (reported once, after the loop)
+  auto fetch_idx = [&](int64_t s) {
+    Idx x;
     const int64_t q = 4 * s + kq;
-    double r = 0, rd = 0;
-    int64_t l = 0;
-    bool ok = q < p;
-    if (ok) {
-      l = sel_idx[p0 + q];
-      if (l < 0 || l >= m) {
-        atomicOr(err, OAKG_DERR_IDX);
-        ok = false;
-        l = 0;
-      } else {
-        const double w = sel_w[p0 + q];
-        r = (w * w) * rho[l];
-        rd = r * innov[l];
-      }
-    }
-    // columns 16 J + jl, J = 0..7, stored contiguously (hs_col<128>)
-    double bv[8];
-    const TS* row = HS + (size_t)l * P128 + 8 * jl;
+    x.ok = q < p;
+    const int64_t qc = x.ok ? q : p - 1;  // p >= 1 here
+    x.l = sel_idx[p0 + qc];
+    x.w = sel_w[p0 + qc];
+    return x;
+  };
+  // columns 16 J + jl, J = 0..7, stored contiguously (hs_col<128>); row 0
+  // stands in for a step past the zone's rows (its values are dropped)
+  auto fetch_row = [&](Idx& x, double& rh, double& inn, TS* rv) {
+    const bool inr = x.l >= 0 && x.l < m;
+    bad = bad | (x.ok & !inr);
+    x.ok = x.ok & inr;
+    x.l = x.ok ? x.l : 0;
+    rh = rho[x.l];
+    inn = innov[x.l];
+    const TS* row = HS + (size_t)x.l * P128 + 8 * jl;
     if constexpr (sizeof(TS) == 4) {
       const float4 u0 = reinterpret_cast<const float4*>(row)[0];
       const float4 u1 = reinterpret_cast<const float4*>(row)[1];
-      bv[0] = u0.x; bv[1] = u0.y; bv[2] = u0.z; bv[3] = u0.w;
-      bv[4] = u1.x; bv[5] = u1.y; bv[6] = u1.z; bv[7] = u1.w;
+      rv[0] = u0.x; rv[1] = u0.y; rv[2] = u0.z; rv[3] = u0.w;
+      rv[4] = u1.x; rv[5] = u1.y; rv[6] = u1.z; rv[7] = u1.w;
     } else {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
         const double2 u = reinterpret_cast<const double2*>(row)[h];
-        bv[2 * h] = u.x;
-        bv[2 * h + 1] = u.y;
+        rv[2 * h] = u.x;
+        rv[2 * h + 1] = u.y;
       }
     }
+  };
+  Idx i0 = fetch_idx(0), i1 = fetch_idx(1);
+  double rh0, inn0;
+  TS rv0[8];
+  fetch_row(i0, rh0, inn0, rv0);
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const Idx i2 = fetch_idx(s + 2);
+    double rh1, inn1;
+    TS rv1[8];
+    fetch_row(i1, rh1, inn1, rv1);
+    // a compiler barrier (no instruction): without it the loads above, whose
+    // only use is the next iteration, are folded into one load at the top
+    // of the loop and the pipeline is gone
+    asm volatile("" ::: "memory");
+    // step s
+    const bool ok = i0.ok;
+    const double r = ok ? (i0.w * i0.w) * rh0 : 0.0;
+    const double rd = ok ? r * inn0 : 0.0;
+    double bv[8];
 #pragma unroll
-    for (int J = 0; J < 8; ++J) bv[J] = ok ? bv[J] : 0.0;
+    for (int J = 0; J < 8; ++J) bv[J] = ok ? (double)rv0[J] : 0.0;
     const double aA = r * bv[IA], aB = r * bv[IB];
 #pragma unroll
     for (int J = 0; J <= IA; ++J) accA[J] = mfma_f64(aA, bv[J], accA[J]);
@@ -89,7 +120,14 @@ __device__ __forceinline__ void gram128_wave(int64_t p0, int64_t p, const int32_
 #pragma unroll
       for (int J = 0; J < 8; ++J) bacc[J] = fma(rd, bv[J], bacc[J]);
     }
+    i0 = i1;
+    i1 = i2;
+    rh0 = rh1;
+    inn0 = inn1;
+#pragma unroll
+    for (int J = 0; J < 8; ++J) rv0[J] = rv1[J];
   }
+  if (bad) atomicOr(err, OAKG_DERR_IDX);
   auto store = [&](int I, int J, const d4& acc) {
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {

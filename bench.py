@@ -65,11 +65,13 @@ def flops_model(N, counts, zone_sizes, dual_lt=0):
     observation counts p_z (zones with p_z > 0), counted as the kernels
     compute them; with the dual bucket on (N > 64, 1 <= p_z < dual_lt) those
     zones count the p x p path instead of the Gram and the eigensolve:
-      gram   2 PD^2 N                 (G = S~ S~^T, PD = 64 padded rows)
+      gram   PD (PD + 1) N            (G = S~ S~^T, PD = 64 padded rows)
       eig    9 PD^3 + 2 N PD p + 4 N p (the 64 x 64 eigensolve and the lift
                                        W = S~^T U c, a, M^1/2 1)
     and otherwise:
-      gram   2 N^2 p + 2 N p          (S^T R^-1 S and S^T R^-1 d)
+      gram   N (N + 1) p + 2 N p      (S^T R^-1 S, a symmetric rank-p update
+                                       counted as BLAS dsyrk is, and
+                                       S^T R^-1 d)
       eig    9 N^3                    (symmetric eigensolve with vectors,
                                        Golub & Van Loan's count for
                                        tridiagonalisation + QL)
@@ -88,7 +90,7 @@ def flops_model(N, counts, zone_sizes, dual_lt=0):
     nact = float(act.sum())
     PD = 64.0
     return {
-        "gram": float(np.sum(2.0 * N * N * p + 2.0 * N * p)) + 2.0 * PD * PD * N * len(pd),
+        "gram": float(np.sum(N * (N + 1.0) * p + 2.0 * N * p)) + PD * (PD + 1.0) * N * len(pd),
         "eig": 9.0 * N ** 3 * float(prim.sum())
                + float(np.sum(9.0 * PD ** 3 + 2.0 * N * PD * pd + 4.0 * N * pd)),
         "trans": (N * N * (N + 1.0) + 10.0 * N * N) * nact,

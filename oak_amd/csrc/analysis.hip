@@ -173,24 +173,53 @@ __global__ __launch_bounds__(256) void k_gram(int64_t z0, const int32_t* __restr
 #pragma unroll
   for (int J = 0; J < NB; ++J) bacc[J] = 0;
   const int64_t nsteps = (p + 3) >> 2;
-  for (int64_t s = wave; s < nsteps; s += 4) {
+  // The steps of a wave (s = wave, wave + 4, ...) are software-pipelined as
+  // in gram128_wave (analysis128.hip): the index and weight two steps ahead
+  // and the gathers of the next step are in flight while a step's MFMAs run;
+  // the step is branch-free, and a compiler barrier keeps the prefetched
+  // loads from being folded into the next iteration.
+  struct Idx {
+    int64_t l;
+    double w;
+    bool ok;
+  };
+  bool bad = false;  // an index outside 0..m-1 (reported once, after the loop)
+  auto fetch_idx = [&](int64_t s) {
+    Idx x;
     const int64_t q = 4 * s + kq;
-    double r = 0, rd = 0;
-    int64_t l = 0;
-    bool ok = q < p;
-    if (ok) {
-      l = sel_idx[p0 + q];
-      if (l < 0 || l >= m) { atomicOr(err, OAKG_DERR_IDX); ok = false; l = 0; }
-      else {
-        double w = sel_w[p0 + q];
-        r = (w * w) * rho[l];
-        rd = r * innov[l];
-      }
-    }
-    double bv[NB];
-    const TS* row = HS + (size_t)l * NP + jl;
+    x.ok = q < p;
+    const int64_t qc = x.ok ? q : p - 1;  // p >= 1 here
+    x.l = sel_idx[p0 + qc];
+    x.w = sel_w[p0 + qc];
+    return x;
+  };
+  auto fetch_row = [&](Idx& x, double& rh, double& inn, TS* rv) {
+    const bool inr = x.l >= 0 && x.l < m;
+    bad = bad | (x.ok & !inr);
+    x.ok = x.ok & inr;
+    x.l = x.ok ? x.l : 0;
+    rh = rho[x.l];
+    inn = innov[x.l];
+    const TS* row = HS + (size_t)x.l * NP + jl;
 #pragma unroll
-    for (int J = 0; J < NB; ++J) bv[J] = ok ? (double)row[16 * J] : 0.0;
+    for (int J = 0; J < NB; ++J) rv[J] = row[16 * J];
+  };
+  Idx i0 = fetch_idx(wave), i1 = fetch_idx(wave + 4);
+  double rh0, inn0;
+  TS rv0[NB];
+  fetch_row(i0, rh0, inn0, rv0);
+  for (int64_t s = wave; s < nsteps; s += 4) {
+    const Idx i2 = fetch_idx(s + 8);
+    double rh1, inn1;
+    TS rv1[NB];
+    fetch_row(i1, rh1, inn1, rv1);
+    asm volatile("" ::: "memory");  // compiler barrier: keeps the loads above here
+    const bool ok = i0.ok;
+    const double r = ok ? (i0.w * i0.w) * rh0 : 0.0;
+    const double rd = ok ? r * inn0 : 0.0;
+    double bv[NB];
+#pragma unroll
+    for (int J = 0; J < NB; ++J) bv[J] = ok ? (double)rv0[J] : 0.0;
     int t = 0;
 #pragma unroll
     for (int I = 0; I < NB; ++I) {
@@ -203,7 +232,14 @@ __global__ __launch_bounds__(256) void k_gram(int64_t z0, const int32_t* __restr
     }
 #pragma unroll
     for (int J = 0; J < NB; ++J) bacc[J] = fma(rd, bv[J], bacc[J]);
+    i0 = i1;
+    i1 = i2;
+    rh0 = rh1;
+    inn0 = inn1;
+#pragma unroll
+    for (int J = 0; J < NB; ++J) rv0[J] = rv1[J];
   }
+  if (bad) atomicOr(err, OAKG_DERR_IDX);
   // b: sum the four k-groups of the wave, then the waves
 #pragma unroll
   for (int J = 0; J < NB; ++J) {

@@ -68,6 +68,22 @@ __global__ void k_state_mean(int64_t n, int N, const double* __restrict__ E, int
   xbar[i] = s / N;
 }
 
+// The same with two consecutive elements per thread (16-byte loads; E 16-byte
+// aligned, ld and n even): half the load instructions, the same sums.
+__global__ void k_state_mean_x2(int64_t n2, int N, const double2* __restrict__ E2, int64_t ld2,
+                                double* __restrict__ xbar) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  double s0 = 0, s1 = 0;
+  for (int k = 0; k < N; ++k) {
+    const double2 v = E2[i + (int64_t)k * ld2];
+    s0 = s0 + v.x;
+    s1 = s1 + v.y;
+  }
+  reinterpret_cast<double2*>(xbar)[i] = make_double2(s0 / N, s1 / N);
+}
+
 // Ensemble mode, HE(:,k) = H.x.E(:,k) + Hshift (assimilation.F90:3112-3114;
 // COO accumulation order of matoper_inc.F90:238-240 kept within each row).
 __global__ void k_spmm_obs(int64_t m, int64_t n, int N, const int64_t* __restrict__ row_ptr,
@@ -676,9 +692,14 @@ void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st) {
 
 template <int NP>
 static void prep_ens_t(const PrepEnsArgs& A, hipStream_t st) {
-  if (A.xbar)  // NULL: observation side only (the report)
-    hipLaunchKernelGGL(k_state_mean, dim3(nblk(A.n, 256)), dim3(256), 0, st, A.n, A.N, A.E,
-                       A.ldE, A.xbar);
+  if (A.xbar) {  // NULL: observation side only (the report)
+    if ((uintptr_t)A.E % 16 == 0 && A.ldE % 2 == 0 && A.n % 2 == 0)
+      hipLaunchKernelGGL(k_state_mean_x2, dim3(nblk(A.n / 2, 256)), dim3(256), 0, st, A.n / 2, A.N,
+                         reinterpret_cast<const double2*>(A.E), A.ldE / 2, A.xbar);
+    else
+      hipLaunchKernelGGL(k_state_mean, dim3(nblk(A.n, 256)), dim3(256), 0, st, A.n, A.N, A.E,
+                         A.ldE, A.xbar);
+  }
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs, dim3(nblk(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E, A.ldE, A.Hshift, A.HEc, A.err);

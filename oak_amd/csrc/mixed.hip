@@ -44,6 +44,26 @@ __global__ void k_state_mean_f32(int64_t n, int N, const float* __restrict__ E, 
   xbar[i] = s / N;
 }
 
+// The same with four consecutive elements per thread (16-byte loads; E 16-byte
+// aligned, ld and n multiples of 4): a quarter of the load instructions, the
+// same sums.
+__global__ void k_state_mean_f32x4(int64_t n4, int N, const float4* __restrict__ E4, int64_t ld4,
+                                   double* __restrict__ xbar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int k = 0; k < N; ++k) {
+    const float4 v = E4[i + (int64_t)k * ld4];
+    s0 += (double)v.x;
+    s1 += (double)v.y;
+    s2 += (double)v.z;
+    s3 += (double)v.w;
+  }
+  double2* x2 = reinterpret_cast<double2*>(xbar + 4 * i);
+  x2[0] = make_double2(s0 / N, s1 / N);
+  x2[1] = make_double2(s2 / N, s3 / N);
+}
+
 __global__ void k_spmm_obs_f32(int64_t m, int64_t n, int N, const int64_t* __restrict__ row_ptr,
                                const int32_t* __restrict__ col, const double* __restrict__ val,
                                const float* __restrict__ E, int64_t ld,
@@ -180,9 +200,14 @@ static inline unsigned nblk32(int64_t n, int b) { return (unsigned)((n + b - 1) 
 
 template <int NP>
 static void prep_ens_f32_t(const PrepEnsArgs& A, hipStream_t st) {
-  if (A.xbar)  // NULL: observation side only (the report)
-    hipLaunchKernelGGL(k_state_mean_f32, dim3(nblk32(A.n, 256)), dim3(256), 0, st, A.n, A.N,
-                       A.E32, A.ldE, A.xbar);
+  if (A.xbar) {  // NULL: observation side only (the report)
+    if ((uintptr_t)A.E32 % 16 == 0 && A.ldE % 4 == 0 && A.n % 4 == 0)
+      hipLaunchKernelGGL(k_state_mean_f32x4, dim3(nblk32(A.n / 4, 256)), dim3(256), 0, st, A.n / 4,
+                         A.N, reinterpret_cast<const float4*>(A.E32), A.ldE / 4, A.xbar);
+    else
+      hipLaunchKernelGGL(k_state_mean_f32, dim3(nblk32(A.n, 256)), dim3(256), 0, st, A.n, A.N,
+                         A.E32, A.ldE, A.xbar);
+  }
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs_f32, dim3(nblk32(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E32, A.ldE, A.Hshift32, A.HEc, A.err);

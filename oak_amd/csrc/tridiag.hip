@@ -979,7 +979,7 @@ __global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, const int32_t*
                                                        const double* __restrict__ dvec, int nzc,
                                                        double* __restrict__ vec,
                                                        double* __restrict__ ampl,
-                                                       unsigned* __restrict__ err) {
+                                                       unsigned* __restrict__ err, int refine) {
   constexpr int NP = 128, H = 64, LD = 65;
   __shared__ Apply128Lds S;
   double* Z = S.z + 8;
@@ -1212,32 +1212,38 @@ __global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, const int32_t*
     av[tid] = a0;
   }
   __syncthreads();
-  // refinement a = a0 + U diag(1/mu) U^T (b - M a0); M symmetric: row r of
-  // M a0 from column r, half of the columns per group (32 loads in flight)
-  {
-    const double* M = mat + (size_t)zl * NP * NP;
-    double mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // refinement a = a0 + U diag(1/mu) U^T (b - M a0) (FP64 path; the mixed
+  // path's bound of 1e-4 is far above the cross-talk it removes); M
+  // symmetric: row r of M a0 from column r, half of the columns per group
+  // (32 loads in flight)
+  double ar = a0;
+  if (refine) {
+    {
+      const double* M = mat + (size_t)zl * NP * NP;
+      double mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int c0 = 0; c0 < H; c0 += 32) {
-      double mv[32];
+      for (int c0 = 0; c0 < H; c0 += 32) {
+        double mv[32];
 #pragma unroll
-      for (int u = 0; u < 32; ++u) mv[u] = M[(size_t)(H * g + c0 + u) * NP + r];
+        for (int u = 0; u < 32; ++u) mv[u] = M[(size_t)(H * g + c0 + u) * NP + r];
 #pragma unroll
-      for (int u = 0; u < 32; ++u) mq[u & 7] = fma(mv[u], av[H * g + c0 + u], mq[u & 7]);
+        for (int u = 0; u < 32; ++u) mq[u & 7] = fma(mv[u], av[H * g + c0 + u], mq[u & 7]);
+      }
+      S.hd[0][g][r] = ((mq[0] + mq[1]) + (mq[2] + mq[3])) + ((mq[4] + mq[5]) + (mq[6] + mq[7]));
     }
-    S.hd[0][g][r] = ((mq[0] + mq[1]) + (mq[2] + mq[3])) + ((mq[4] + mq[5]) + (mq[6] + mq[7]));
+    __syncthreads();
+    if (tid < NP) av[tid] = bv[tid] - (S.hd[0][0][tid] + S.hd[0][1][tid]);
+    __syncthreads();
+    colsum1(H);  // upper half still staged
+    stage(0);
+    colsum1(0);
+    rowdot_h(xv, nullptr);
+    __syncthreads();
+    if (tid < NP) ar = a0 + (S.hd[0][0][tid] + S.hd[0][1][tid]);
+  } else {
+    stage(0);  // the lower half, for Y below
   }
-  __syncthreads();
-  if (tid < NP) av[tid] = bv[tid] - (S.hd[0][0][tid] + S.hd[0][1][tid]);
-  __syncthreads();
-  colsum1(H);  // upper half still staged
-  stage(0);
-  colsum1(0);
-  rowdot_h(xv, nullptr);
-  __syncthreads();
-  double ar = 0;
   if (tid < NP) {
-    ar = a0 + (S.hd[0][0][tid] + S.hd[0][1][tid]);
     av[tid] = vr * vr;
     yv[tid] = 1.0 / sqrt(sqrt(mu[tid]));
   }
@@ -1415,7 +1421,7 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   if constexpr (NP == 128) {  // the epilogue runs at the end of the apply kernel
     hipLaunchKernelGGL(k_tql_apply128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr, A.mat,
                        A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,
-                       A.vec, A.ampl, A.err);
+                       A.vec, A.ampl, A.err, A.prec == 1 ? 0 : 1);
   } else {
     hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0, A.zorder,
                        A.sel_ptr, A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status);

@@ -58,6 +58,7 @@ extern "C" {
 #define OAKG_EHIP 3   /* HIP runtime error                                          */
 #define OAKG_ENOMEM 4 /* device allocation failed                                   */
 #define OAKG_EUNSUP 5 /* configuration not supported (e.g. N above the maximum)     */
+#define OAKG_ECONV 6  /* eigensolver did not finish a zone (N > 128: no fallback)   */
 
 #define OAKG_FP64 0  /* plain FP64 path                                             */
 #define OAKG_MIXED 1 /* FP32/BF16 contractions, FP64 eigensolve                     */
@@ -68,7 +69,7 @@ extern "C" {
 #define OAKG_SPHERICAL 1
 #define OAKG_SPHERICAL_APPROX 2
 
-#define OAKG_MAX_N 128 /* largest ensemble size of this build */
+#define OAKG_MAX_N 256 /* largest ensemble size of this build (the likelihood: 128) */
 
 typedef struct oakg_ctx oakg_ctx;
 

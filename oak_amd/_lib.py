@@ -12,10 +12,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboakg.so")
 
-OAKG_OK, OAKG_EARG, OAKG_ENAN, OAKG_EHIP, OAKG_ENOMEM, OAKG_EUNSUP = range(6)
+OAKG_OK, OAKG_EARG, OAKG_ENAN, OAKG_EHIP, OAKG_ENOMEM, OAKG_EUNSUP, OAKG_ECONV = range(7)
 OAKG_FP64, OAKG_MIXED = 0, 1
 OAKG_DEVICE_PTRS = 1
-OAKG_MAX_N = 128
+OAKG_MAX_N = 256
 OAKG_NOALLOC_SCRATCH_BYTES = 256 << 20
 OAKG_EIG_JACOBI, OAKG_EIG_QL = 0, 1
 

@@ -674,7 +674,7 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 
 template <int NP>
 static void prep_loc_t(const PrepLocArgs& A, hipStream_t st) {
-  constexpr int TR = NP > 64 ? 32 : 64;
+  constexpr int TR = NP > 128 ? 16 : (NP > 64 ? 32 : 64);
   hipLaunchKernelGGL((k_prep_loc<NP, TR>), dim3(nblk(A.m, TR)), dim3(256), 0, st, A.m, A.N, A.HSf,
                      A.ldHSf, A.Hxf, A.yo, A.rmse2, A.excl, A.HS, A.innov, A.rho, A.err,
                      A.hs_scale);
@@ -687,6 +687,7 @@ void launch_prep_loc(int NP, const PrepLocArgs& A, hipStream_t st) {
     case 32: prep_loc_t<32>(A, st); break;
     case 64: prep_loc_t<64>(A, st); break;
     case 128: prep_loc_t<128>(A, st); break;
+    case 256: prep_loc_t<256>(A, st); break;
   }
 }
 
@@ -703,7 +704,7 @@ static void prep_ens_t(const PrepEnsArgs& A, hipStream_t st) {
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs, dim3(nblk(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E, A.ldE, A.Hshift, A.HEc, A.err);
-  constexpr int TR = NP > 64 ? 32 : 64;
+  constexpr int TR = NP > 128 ? 16 : (NP > 64 ? 32 : 64);
   hipLaunchKernelGGL((k_obs_anom<NP, TR>), dim3(nblk(A.m, TR)), dim3(256), 0, st, A.m, A.N, A.HEc, A.yo,
                      A.rmse2, A.excl, A.scaling, A.HS, A.innov, A.rho, A.err);
 }
@@ -718,6 +719,7 @@ void launch_prep_ens(int NP, const PrepEnsArgs& A, hipStream_t st) {
     case 32: prep_ens_t<32>(A, st); break;
     case 64: prep_ens_t<64>(A, st); break;
     case 128: prep_ens_t<128>(A, st); break;
+    case 256: prep_ens_t<256>(A, st); break;
   }
 }
 
@@ -787,6 +789,7 @@ void launch_chunk(int NP, const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
     case 32: chunk_t<32>(A, st, tm); break;
     case 64: chunk_t<64>(A, st, tm); break;
     case 128: launch_chunk128(A, st, tm); break;
+    case 256: launch_chunk256(A, st, tm); break;
   }
 }
 

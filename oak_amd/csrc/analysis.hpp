@@ -90,6 +90,7 @@ struct ChunkArgs {
   const double* sel_w;
   const double *rho, *innov, *HS;
   double *mat, *ymat, *vec, *ampl;
+  double* umat;  // NP = 256: Q, then U, then Y (mat: M then T; ymat: the reflectors)
   int32_t* sweeps;
   // eigensolver: 0 = one-sided Jacobi, 1 = Householder tridiagonalisation + implicit QL
   int eig_method;
@@ -166,6 +167,10 @@ void launch_dual128(const ChunkArgs& A, int N, double* ymat128, double* vec128, 
                     hipStream_t st, PhaseTimer* tm);
 void launch_chunk128(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
+// NP = 256 (wide256.hip): the whole chunk; QL, rotations and epilogue of
+// its eigensolver (tridiag.hip)
+void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
+void launch_ql256(const ChunkArgs& A, hipStream_t st);
 void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_nll_adjoint(int NP, const ChunkArgs& A, hipStream_t st);

@@ -253,6 +253,7 @@ int padded_n(int N) {
   if (N <= 32) return 32;
   if (N <= 64) return 64;
   if (N <= 128) return 128;
+  if (N <= 256) return 256;
   return 0;
 }
 
@@ -408,6 +409,10 @@ void finish_call(oakg_ctx* c, int64_t nz, const int64_t* sel_ptr, const int32_t*
   const unsigned herr = (unsigned)h[0];
   if (herr & OAKG_DERR_RMSE) arg_fail(c, "observation error variance rmse^2 <= 0 or not finite");
   if (herr & OAKG_DERR_IDX) arg_fail(c, "index out of range in H or in the selection");
+  if (herr & OAKG_DERR_NOCONV) {
+    c->err = "the QL iteration did not finish a zone (no fallback above N = 128)";
+    throw Fail{OAKG_ECONV};
+  }
   if (herr & OAKG_DERR_NAN) {
     c->err = "NaN in the analysis amplitudes (rrsqrt.F90:145-149)";
     throw Fail{OAKG_ENAN};
@@ -458,12 +463,12 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   // interleaved in groups of 4 records over the 64 zones of a QL wave)
   // (about 1.1 N^2 records for typical zones; 2 N^2 + 8 N leaves room for slow ones)
   const int64_t cap = (((int64_t)2 * NP * NP + 8 * NP) + 3) & ~(int64_t)3;
-  const bool ql = c->eig_method == 1;
+  const bool ql = c->eig_method == 1 || NP == 256;  // N > 128: QL only
   // zones with 1 <= p < dual_lt in observation space (dual.hip): N > 64, the
   // analysis modes (not the likelihood), not in a no-allocation call
   const int dual_lt = (NP == 128 && base.mode != 2 && !c->arena && nz > 0) ? c->tune_dual : 0;
   constexpr int PD = 64;
-  const size_t per_zone = sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) +
+  const size_t per_zone = sizeof(double) * ((size_t)(NP == 256 ? 3 : 2) * NP * NP + 6 * NP) +
                           (ql ? (size_t)cap * 16 + 4 : 0) +
                           (dual_lt ? sizeof(double) * ((size_t)2 * PD * PD + 4 * PD) : 0);
   const size_t budget = c->ws_budget;
@@ -488,7 +493,9 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
   double* mat = (double*)wsget(c, Workspace::MAT, sizeof(double) * (size_t)NP * NP * zs);
   double* ymat = (double*)wsget(c, Workspace::YMAT, sizeof(double) * (size_t)NP * NP * zs);
   double* vec = (double*)wsget(c, Workspace::VEC, sizeof(double) * (size_t)4 * NP * zs);
-  if (!mat || !ymat || !vec) check(c, hipErrorOutOfMemory, "workspace");
+  double* umat = nullptr;  // NP = 256: Q, U, Y (the reflectors stay in ymat)
+  if (NP == 256) umat = (double*)wsget(c, Workspace::UMAT, sizeof(double) * (size_t)NP * NP * zs);
+  if (!mat || !ymat || !vec || (NP == 256 && !umat)) check(c, hipErrorOutOfMemory, "workspace");
   double *dvec = nullptr, *evec = nullptr;
   char* rlog = nullptr;
   int32_t* tst = nullptr;
@@ -550,7 +557,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
       ChunkArgs base;
       int64_t nz, zc, cap;
       int NP, nslot, ql, ql_zw, order, tri_split, dual_lt, persist;
-      const void* buf[15];
+      const void* buf[16];
       hipStream_t stream;
     } kk;
     memset(&kk, 0, sizeof kk);
@@ -566,8 +573,8 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     kk.tri_split = c->tune_tri_split;
     kk.dual_lt = dual_lt;
     kk.persist = persist ? 1 + persist_item : 0;
-    const void* bufs[15] = {mat,  ymat,   vec,    dvec,   evec,  rlog,  tst,   sweeps_all,
-                            zorder, zord_p, zord_d, mat_d, ymat_d, vec_d, wq};
+    const void* bufs[16] = {mat,  ymat,   vec,    dvec,   evec,  rlog,  tst,   sweeps_all,
+                            zorder, zord_p, zord_d, mat_d, ymat_d, vec_d, wq,  umat};
     memcpy(kk.buf, bufs, sizeof bufs);
     kk.stream = c->stream;
     key.assign((const char*)&kk, (const char*)&kk + sizeof kk);
@@ -631,6 +638,7 @@ void run_zones(oakg_ctx* c, int NP, ChunkArgs base, int64_t nz, int32_t* sweeps_
     A.mat = mat + zo * NP * NP;
     A.ymat = ymat + zo * NP * NP;
     A.vec = vec + zo * 4 * NP;
+    A.umat = umat ? umat + zo * NP * NP : nullptr;
     A.sweeps = sweeps_all + z0;
     A.eig_method = ql ? 1 : 0;
     A.dvec = ql ? dvec + zo * NP : nullptr;
@@ -712,7 +720,7 @@ int local_analysis_impl(oakg_ctx* c, const oakg_zones* zones, int64_t nzones_csr
   if (N < 1 || N > OAKG_MAX_N) arg_fail(c, "N out of range");
   const int NP = padded_n(N);
   if (!NP) {
-    c->err = "N above 128 is not supported by this build";
+    c->err = "N above 256 is not supported by this build";
     return OAKG_EUNSUP;
   }
   if (!csr) validate_zones_host(c, zones, 0, true);
@@ -855,10 +863,10 @@ int local_nll_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, in
                    double* nll, double* grad_log_sigma, double* grad_corr_len,
                    double* grad_inflation, double* nll_zone, uint32_t flags) {
   const bool dev = flags & OAKG_DEVICE_PTRS;
-  if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+  if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..256)");
   const int NP = padded_n(N);
-  if (!NP) {
-    c->err = "N above 128 is not supported by this build";
+  if (!NP || NP > 128) {
+    c->err = "the likelihood supports N up to 128";
     return OAKG_EUNSUP;
   }
   validate_zones_host(c, zones, 0, true);
@@ -950,10 +958,10 @@ int ensemble_impl(oakg_ctx* c, const oakg_zones* zones, const oakg_obs* obs, con
   return guarded(c, [&] {
     const bool dev = flags & OAKG_DEVICE_PTRS;
     if (!zones || !obs || !H || !Ef || !Ea) arg_fail(c, "zones, obs, H, Ef and Ea are required");
-    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..256)");
     const int NP = padded_n(N);
     if (!NP) {
-      c->err = "N above 128 is not supported by this build";
+      c->err = "N above 256 is not supported by this build";
       return OAKG_EUNSUP;
     }
     validate_zones_host(c, zones, 0, true);
@@ -1153,7 +1161,7 @@ int report_impl(oakg_ctx* c, int64_t m, int32_t N, const double* yo_Hx, const do
   if (m < 0 || N < 1) arg_fail(c, "m must be >= 0 and N >= 1");
   const int NP = padded_n(N);
   if (!NP) {
-    c->err = "N above 128 is not supported by this build";
+    c->err = "N above 256 is not supported by this build";
     return OAKG_EUNSUP;
   }
   if (!stats) arg_fail(c, "stats is required");
@@ -1186,7 +1194,7 @@ int ensemble_report_impl(oakg_ctx* c, const oakg_obs* obs, const oakg_csr* H, co
   return guarded(c, [&] {
     const bool dev = flags & OAKG_DEVICE_PTRS;
     if (!obs || !H || !E || !stats) arg_fail(c, "obs, H, E and stats are required");
-    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..128)");
+    if (N < 2 || N > OAKG_MAX_N) arg_fail(c, "N out of range (2..256)");
     const int NP = padded_n(N);
     validate_obs(c, obs, false, 1);
     const int64_t m = obs->m, n = H->n;
@@ -1528,7 +1536,8 @@ size_t noalloc_fixed_bytes(int64_t m, int NP, int64_t nz) {
 // Bytes of the zone-chunk workspace per zone (as run_zones sizes it).
 size_t noalloc_zone_bytes(int NP, bool ql) {
   const int64_t cap = (((int64_t)2 * NP * NP + 8 * NP) + 3) & ~(int64_t)3;
-  return sizeof(double) * ((size_t)2 * NP * NP + 6 * NP) + (ql ? (size_t)cap * 16 + 4 : 0);
+  return sizeof(double) * ((size_t)(NP == 256 ? 3 : 2) * NP * NP + 6 * NP) +
+         (ql || NP == 256 ? (size_t)cap * 16 + 4 : 0);
 }
 
 // For the duration of a no-allocation call: the arena over the scratch, the

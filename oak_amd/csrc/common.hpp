@@ -11,6 +11,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 #define OAKG_DERR_NAN 2u    // NaN amplitude (rrsqrt.F90:145-149)
 #define OAKG_DERR_CAP 4u    // selection list overflow
 #define OAKG_DERR_IDX 8u    // observation/state index out of range
+#define OAKG_DERR_NOCONV 16u  // QL did not finish a zone that has no fallback (N > 128)
 
 // v_mfma_f64_16x16x4f64 operand layout on gfx950 (checked on the device):
 //   A (16x4):  lane l holds A[l % 16][l / 16]

@@ -44,7 +44,7 @@ class Workspace {
   enum Slot {
     // internal buffers
     COUNTS, SEL_PTR, SEL_IDX, SEL_W, ZORDER, ZORDER_P, ZORDER_D, PERSIST_WQ, MAT_D, YMAT_D, VEC_D, GRID, KEY, KEY2, IDX, SIDX, CELL_CNT, CELL_PTR, SX, SY,
-    SORT_TMP, SCAN_TMP, SCAN_TMP2, HS, INNOV, RHO, XBAR, HEC, MAT, YMAT, VEC, SWEEPS, DVEC, EVEC, RLOG, TQL_STATUS, HS32, NLL_Z, NLL_GTH,
+    SORT_TMP, SCAN_TMP, SCAN_TMP2, HS, INNOV, RHO, XBAR, HEC, MAT, YMAT, VEC, SWEEPS, DVEC, EVEC, RLOG, UMAT, TQL_STATUS, HS32, NLL_Z, NLL_GTH,
     // staged copies of host inputs and outputs
     Z_PTR, Z_POS, Z_CORR, Z_MAXLEN, O_X, O_Y, O_Z, O_T, O_YO, O_RMSE2, O_EXCL, IN_S, IN_HSF,
     IN_XF, IN_HXF, H_RP, H_COL, H_VAL, H_SHIFT, OUT_XA, OUT_S, OUT_AMPL, U_SEL_PTR, U_SEL_IDX,

@@ -211,7 +211,7 @@ static void prep_ens_f32_t(const PrepEnsArgs& A, hipStream_t st) {
   if (A.m == 0) return;
   hipLaunchKernelGGL(k_spmm_obs_f32, dim3(nblk32(A.m, 256), A.N), dim3(256), 0, st, A.m, A.n, A.N,
                      A.row_ptr, A.col, A.val, A.E32, A.ldE, A.Hshift32, A.HEc, A.err);
-  constexpr int TR = NP > 64 ? 32 : 64;
+  constexpr int TR = NP > 128 ? 16 : (NP > 64 ? 32 : 64);
   hipLaunchKernelGGL((k_obs_anom_f32<NP, TR>), dim3(nblk32(A.m, TR)), dim3(256), 0, st, A.m, A.N,
                      A.HEc, A.yo, A.rmse2, A.excl, A.scaling, A.HS32, A.innov, A.rho, A.err);
 }
@@ -222,6 +222,7 @@ void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st) {
     case 32: prep_ens_f32_t<32>(A, st); break;
     case 64: prep_ens_f32_t<64>(A, st); break;
     case 128: prep_ens_f32_t<128>(A, st); break;
+    case 256: prep_ens_f32_t<256>(A, st); break;
   }
 }
 

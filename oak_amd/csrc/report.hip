@@ -230,6 +230,7 @@ void launch_report(int NP, const ReportArgs& A, hipStream_t st) {
     case 32: report_t<32>(A, st); break;
     case 64: report_t<64>(A, st); break;
     case 128: report_t<128>(A, st); break;
+    case 256: report_t<256>(A, st); break;
   }
 }
 

@@ -27,25 +27,9 @@
 
 #include "analysis.hpp"
 #include "common.hpp"
+#include "eigen_common.hpp"
 
 namespace oakg {
-
-__device__ __forceinline__ double wave_sum_d(double s) { return wave_sum_fast(s); }
-
-// 1/sqrt(h) to full double precision: the hardware estimate y (relative
-// error below 2^-22) and one third-order correction, y (1 + e/2 + 3e^2/8)
-// with e = 1 - h y^2 (the neglected term 5e^3/16 is below 2^-64)
-__device__ __forceinline__ double rsqrt_acc(double h) {
-  const double y = __builtin_amdgcn_rsq(h);
-  const double e = fma(-h, y * y, 1.0);
-  return fma(y * e, fma(0.375, e, 0.5), y);
-}
-// 1/x the same way: y (1 + e + e^2) with e = 1 - x y
-__device__ __forceinline__ double rcp_acc(double x) {
-  const double y = __builtin_amdgcn_rcp(x);
-  const double e = fma(-x, y, 1.0);
-  return fma(fma(e, e, e), y, y);
-}
 
 // ---------------------------------------------------------------------------
 // Householder tridiagonalisation (lower, as dsytd2) with the matrix in
@@ -73,7 +57,6 @@ __device__ __forceinline__ double rcp_acc(double x) {
 // tri_rx(r) = r + r / 8: the tiles of the first stage start every 8 rows,
 // and without the pad the reads of v and p by the 16 tile rows of a wave
 // fall four to a bank.
-__host__ __device__ constexpr int tri_rx(int r) { return r + (r >> 3); }
 
 // XF: side of the largest trailing block moved between stages (NP/2 from
 // the first stage, NP/4 when a launch starts at the second).
@@ -673,17 +656,6 @@ __global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int32_t* 
         V[(size_t)(16 * I + lr + 4 * r) * NP + (NP - 1 - (16 * (jj ? J1 : J0) + lc))] = q[jj][I][r];
 }
 
-// ---------------------------------------------------------------------------
-// Rotation log.  The zones of one 64-zone group share cap * 64 records; record
-// k of the zone in lane l sits at (k / 4) * 256 + 4 l + (k % 4), so that one
-// store instruction of the QL wave stays within a few KB.  A chain (one QL
-// sweep on rows l..m) starts with a header {l + 65536 m, count} followed by
-// `count` records {c, s} for the rotation indices i = m-1, m-2, ...
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ size_t rlog_off(int zl, int64_t cap) {
-  return (size_t)(zl >> 6) * (size_t)cap * 64 + (size_t)(zl & 63) * 4;
-}
-__device__ __forceinline__ size_t rlog_rec(int64_t k) { return (size_t)(k >> 2) * 256 + (size_t)(k & 3); }
 
 
 // Implicit QL with Wilkinson shifts on (d, e), one lane per zone (EISPACK
@@ -930,6 +902,117 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0, con
     b ^= 1;
   }
   for (int e = tid; e < NP * NP; e += BS) Q[e] = Z[(e / NP) * LD + e % NP];
+}
+
+// ---------------------------------------------------------------------------
+// U = Q Z for NP = 256: the rows of Q are independent, so workgroup y applies
+// the logged rotations to rows RB y .. RB y + RB - 1 in LDS (NP = 256 rows
+// would need 512 KB), one thread per row, with the operations of
+// k_tql_apply.  Each thread also stages NP / RB records of the next chain.
+// A zone whose QL did not finish is reported (no fallback at this size).
+// ---------------------------------------------------------------------------
+template <int NP, int RB>
+__global__ __launch_bounds__(RB) void k_tql_apply_rows(int64_t z0, const int32_t* __restrict__ zorder,
+                                                       const int64_t* __restrict__ sel_ptr,
+                                                       double* __restrict__ qmat,
+                                                       const double2* __restrict__ rlog,
+                                                       int64_t cap,
+                                                       const int32_t* __restrict__ status,
+                                                       unsigned* __restrict__ err) {
+  constexpr int LD = NP + 1;
+  constexpr int PAD = 8;  // room for the look-ahead reads below column 0 of row 0
+  constexpr int NQ = NP / RB;
+  __shared__ double Zs[PAD + RB * LD];
+  __shared__ double2 buf[2][NP + 12];
+  double* Z = Zs + PAD;
+  const int zl = blockIdx.x;
+  const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;
+  if (sel_ptr[z + 1] == sel_ptr[z]) return;
+  const int stz = status[zl];
+  const int tid = threadIdx.x;
+  if (stz & 3) {
+    if (blockIdx.y == 0 && tid == 0) atomicOr(err, OAKG_DERR_NOCONV);
+    return;
+  }
+  double* Q = qmat + (size_t)zl * NP * NP + (size_t)blockIdx.y * RB * NP;
+  copy_batched<16>(tid, RB * NP, RB, [&](int e) { return Q[e]; },
+                   [&](int e, double x) { Z[(e / NP) * LD + e % NP] = x; });
+  const int64_t len = stz >> 2;
+  const double2* lg = rlog + rlog_off(zl, cap);
+#pragma unroll
+  for (int u = 0; u < NQ; ++u) {
+    const int q = tid + RB * u;
+    if (q < len) buf[0][q] = lg[rlog_rec(q)];
+  }
+  __syncthreads();
+  int64_t pos = 0;
+  int b = 0;
+  while (pos < len) {
+    const double2 h = buf[b][0];
+    const int m = __builtin_amdgcn_readfirstlane(((int)h.x) >> 16);
+    const int cnt = __builtin_amdgcn_readfirstlane((int)h.y);
+    const int64_t nxt = pos + 1 + cnt;
+    double2 pv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + RB * u;
+      pv[u] = nxt + q < len ? lg[rlog_rec(nxt + q)] : make_double2(0.0, 0.0);
+    }
+    {
+      // column i of this row is p[0]; rotations i = m-1, m-2, ... with
+      // (c, s) = CS(0), CS(1), ... (the loop of k_tql_apply)
+      typedef __attribute__((address_space(3))) const double lds_d;
+      double* p = Z + tid * LD + (m - 1);
+      lds_d* cd = (lds_d*)(buf[b] + 1);
+      asm volatile("" : "+v"(cd));  // one VGPR base for the broadcast reads
+      double carry = p[1];
+      int k = 0;
+#define CS(J) make_double2(cd[2 * (J)], cd[2 * (J) + 1])
+#define OAK_ROT(Q, ZV, DST)                  \
+  DST = fma(Q.y, ZV, Q.x * carry);           \
+  carry = fma(Q.x, ZV, -Q.y * carry);
+      if (cnt >= 8) {
+        double a0 = p[0], a1 = p[-1], a2 = p[-2], a3 = p[-3];
+        double2 qa0 = CS(0), qa1 = CS(1), qa2 = CS(2), qa3 = CS(3);
+        for (; k + 8 <= cnt; k += 8) {
+          const double b0 = p[-4], b1 = p[-5], b2 = p[-6], b3 = p[-7];
+          const double2 qb0 = CS(4), qb1 = CS(5), qb2 = CS(6), qb3 = CS(7);
+          OAK_ROT(qa0, a0, p[1])
+          OAK_ROT(qa1, a1, p[0])
+          OAK_ROT(qa2, a2, p[-1])
+          OAK_ROT(qa3, a3, p[-2])
+          a0 = p[-8]; a1 = p[-9]; a2 = p[-10]; a3 = p[-11];
+          qa0 = CS(8); qa1 = CS(9); qa2 = CS(10); qa3 = CS(11);
+          OAK_ROT(qb0, b0, p[-3])
+          OAK_ROT(qb1, b1, p[-4])
+          OAK_ROT(qb2, b2, p[-5])
+          OAK_ROT(qb3, b3, p[-6])
+          p -= 8;
+          cd += 16;
+        }
+      }
+      for (; k < cnt; ++k) {
+        const double zv = p[0];
+        const double2 q = CS(0);
+        OAK_ROT(q, zv, p[1])
+        p -= 1;
+        cd += 2;
+      }
+#undef OAK_ROT
+#undef CS
+      p[1] = carry;  // column m - cnt
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + RB * u;
+      if (nxt + q < len) buf[b ^ 1][q] = pv[u];
+    }
+    __syncthreads();
+    pos = nxt;
+    b ^= 1;
+  }
+  for (int e = tid; e < RB * NP; e += RB) Q[e] = Z[(e / NP) * LD + e % NP];
 }
 
 // ---------------------------------------------------------------------------
@@ -1428,6 +1511,37 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
     hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
   }
+}
+
+// NP = 256: QL on the chunk's (d, e), the rotations on Q (umat) in 64-row
+// blocks, then the epilogue (a, v, mu and Y in umat).
+void launch_ql256(const ChunkArgs& A, hipStream_t st) {
+  constexpr int NP = 256;
+#define OAK_TQL256(ZW)                                                                                    \
+  do {                                                                                                    \
+    if (A.prec == 1)                                                                                      \
+      hipLaunchKernelGGL((k_tql<NP, ZW, true>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc,     \
+                         A.z0, A.zorder, A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap,         \
+                         A.tql_status, A.err);                                                            \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_tql<NP, ZW, false>), dim3((A.nzc + ZW - 1) / ZW), dim3(64), 0, st, A.nzc,    \
+                         A.z0, A.zorder, A.sel_ptr, A.dvec, A.evec, (double2*)A.rlog, A.rlog_cap,         \
+                         A.tql_status, A.err);                                                            \
+  } while (0)
+  if (A.ql_zw == 4)
+    OAK_TQL256(4);
+  else if (A.ql_zw == 8)
+    OAK_TQL256(8);
+  else if (A.ql_zw == 32)
+    OAK_TQL256(32);
+  else
+    OAK_TQL256(16);
+#undef OAK_TQL256
+  hipLaunchKernelGGL((k_tql_apply_rows<NP, 64>), dim3(A.nzc, NP / 64), dim3(64), 0, st, A.z0,
+                     A.zorder, A.sel_ptr, A.umat, (const double2*)A.rlog, A.rlog_cap,
+                     A.tql_status, A.err);
+  hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
+                     A.mat, A.umat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
 }
 
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st) {

@@ -8,7 +8,7 @@ module oakg
   implicit none
 
   integer(c_int), parameter :: OAKG_OK = 0, OAKG_EARG = 1, OAKG_ENAN = 2, OAKG_EHIP = 3, &
-       OAKG_ENOMEM = 4, OAKG_EUNSUP = 5
+       OAKG_ENOMEM = 4, OAKG_EUNSUP = 5, OAKG_ECONV = 6
   integer(c_int), parameter :: OAKG_FP64 = 0, OAKG_MIXED = 1
   integer(c_int32_t), parameter :: OAKG_DEVICE_PTRS = 1
   integer(c_int), parameter :: OAKG_EIG_JACOBI = 0, OAKG_EIG_QL = 1

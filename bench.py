@@ -34,7 +34,7 @@ PEAK_FP64 = 78.6e12   # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_HBM = 8.0e12     # MI355X HBM3E, B/s
 
 
-def build_problem(cfg, device):
+def build_problem(cfg, device, scale=1):
     from oak_amd import synth
     if cfg == 0:
         import torch
@@ -45,6 +45,11 @@ def build_problem(cfg, device):
         return synth.config1_gpu(device), "360x180x30x4+SSH (n=7,840,800), N=64, 201k obs"
     if cfg == 2:
         return synth.config2_gpu(device=device), "360x180x30x4+SSH (n=7,840,800), N=128, clustered obs"
+    if cfg == 4:
+        nx, ny = 720 // scale, 360 // scale
+        return (synth.config4_gpu(scale, device=device),
+                "%dx%dx40x4+SSH (n=%d), N=256 FP32 members, swath + profile obs, L=3, cutoff 9"
+                % (nx, ny, nx * ny * 161))
     raise SystemExit("unknown config %d" % cfg)
 
 
@@ -298,7 +303,7 @@ def run_report(args, H, Hshift, yo, rmse2, Ef, Ea):
             "analysis": dict(zip(REPORT_FIELDS, ra.tolist()))}
 
 
-def phase_rooflines(F, phase, mixed):
+def phase_rooflines(F, phase, mixed, N=128):
     """Achieved rate of every phase against the MI355X peak of the arithmetic
     it runs on: the Gram (FP64 MFMA accumulation in both paths), the
     eigensolve (FP64), the transform and update (FP32 MFMA on the mixed path,
@@ -311,7 +316,8 @@ def phase_rooflines(F, phase, mixed):
     for k, w in work.items():
         if phase[k] <= 0:
             continue
-        peak = PEAK_FP32_MFMA if (mixed and k in ("update", "tmat")) else PEAK_FP64
+        # N > 128: the transform and update run on FP64 MFMA in both paths
+        peak = PEAK_FP32_MFMA if (mixed and N <= 128 and k in ("update", "tmat")) else PEAK_FP64
         ach = w / (phase[k] * 1e-3)
         out[k] = {"ms": round(phase[k], 3), "flop": w, "achieved": ach / 1e12,
                   "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak}
@@ -328,7 +334,7 @@ def roofline(F, phase, mixed, eig_name, config, N):
     if fused:
         work["update"] += work["tmat"]
     kern = max(work, key=lambda k: phase[k])
-    gk = "k_gram128" if N > 64 else "k_gram"
+    gk = "k_gram256" if N > 128 else ("k_gram128" if N > 64 else "k_gram")
     names = {
         "gram": ("%s<float> (S^T R^-1 S from FP32 S, v_mfma_f64_16x16x4f64)" % gk if mixed else
                  "%s (S^T R^-1 S, v_mfma_f64_16x16x4f64)" % gk),
@@ -337,12 +343,17 @@ def roofline(F, phase, mixed, eig_name, config, N):
         "update": ("k_update_f32 (Ea = xa + A' W, v_mfma_f32_16x16x4f32)" if mixed else
                    "k_update (Ea = xa + A' W, v_mfma_f64_16x16x4f64)"),
     }
+    if N > 128:
+        names["tmat"] = "k_tmat256 (T = Y Y^T Omega, v_mfma_f64_16x16x4f64)"
+        names["update"] = ("k_update256 (Ea = xa + A' T from FP32 members on "
+                           "v_mfma_f64_16x16x4f64)" if mixed else
+                           "k_update256 (Ea = xa + A' T, v_mfma_f64_16x16x4f64)")
     if fused:
         names["update"] = ("k_xform128f (T = Y Y^T Omega in LDS on v_mfma_f32_16x16x4f32, then "
                            "Ea = xa + A' T on v_mfma_f32_16x16x4f32)" if mixed else
                            "k_xform128 (T = Y Y^T Omega in LDS on v_mfma_f64_16x16x4f64, then "
                            "Ea = xa + A' T on v_mfma_f64_16x16x4f64)")
-    peak = PEAK_FP32_MFMA if (mixed and kern == "update") else PEAK_FP64
+    peak = PEAK_FP32_MFMA if (mixed and N <= 128 and kern == "update") else PEAK_FP64
     ach = work[kern] / (phase[kern] * 1e-3)
     return {"bound": "mfma", "kernel": names[kern], "achieved": ach / 1e12, "peak": peak / 1e12,
             "unit": "TFLOP/s", "frac": ach / peak, "traffic": phase_traffic(kern, mixed, config)}
@@ -455,9 +466,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=1,
+                    help="config 4: divide both horizontal grid sizes (and the profiles by "
+                         "scale^2), keeping the local observation counts")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed", "both"],
                     help="default: both for config 2 (the headline: mixed timed against FP64), "
-                         "fp64 otherwise")
+                         "mixed for config 4 (FP32 members), fp64 otherwise")
     ap.add_argument("--eig", default="ql", choices=["ql", "jacobi"],
                     help="eigensolver: tridiagonal QL (default) or one-sided Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
@@ -476,7 +490,7 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    prec = args.precision or ("both" if args.config == 2 else "fp64")
+    prec = args.precision or {2: "both", 4: "mixed"}.get(args.config, "fp64")
     import torch
     from oak_amd import Context, OAKG_FP64, OAKG_MIXED
     from oak_amd._lib import OAKG_EIG_JACOBI, OAKG_EIG_QL
@@ -494,12 +508,15 @@ def main():
         run_config3(args, device)
         return
     t0 = time.time()
-    DP, desc = build_problem(args.config, device)
+    DP, desc = build_problem(args.config, device, args.scale)
     P = DP.host
     g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
     print("# generated %s in %.1f s" % (desc, time.time() - t0), file=sys.stderr, flush=True)
     n, N = DP.E.shape
-    eig_name = (("k_tridiag (two launches)+k_formq128+k_tql+k_tql_apply128 (Householder, Q in "
+    eig_name = (("k_tridiag256+k_formq256+k_tql+k_tql_apply_rows+k_eigvec_epi (Householder on "
+                 "register tiles, Q in blocked WY form on FP64 MFMA, implicit QL, rotations on "
+                 "64-row blocks, eigenvector epilogue; FP64)" if N > 128 else
+                 "k_tridiag (two launches)+k_formq128+k_tql+k_tql_apply128 (Householder, Q in "
                  "blocked WY form on FP64 MFMA, implicit QL, eigenvector epilogue; FP64), with "
                  "the zones of p < 64 through k_tridiag+k_tql+k_tql_apply+k_eigvec_epi at 64 "
                  "and k_dual_lift" if N > 64 else
@@ -545,7 +562,7 @@ def main():
                    "max_p": int(counts.max()),
                    "phase_ms": {k: round(v, 3) for k, v in phase.items()}},
         "roofline": roofline(F, phase, head == "mixed", eig_name, args.config, N),
-        "roofline_phases": phase_rooflines(F, phase, head == "mixed"),
+        "roofline_phases": phase_rooflines(F, phase, head == "mixed", N),
         "launch": res[head][2].get("tuning"),
     }
     if "mixed" in res and "fp64" in res:
@@ -553,7 +570,7 @@ def main():
         F64 = flops_model(N, counts, zs, st64.get("tuning", {}).get("dual", 0))
         out["fp64"] = {"value": n / (ms64 * 1e-3), "ms_per_step": ms64,
                        "phase_ms": {k: round(v, 3) for k, v in ph64.items()},
-                       "roofline_phases": phase_rooflines(F64, ph64, False),
+                       "roofline_phases": phase_rooflines(F64, ph64, False, N),
                        "launch": st64.get("tuning")}
         out["mixed_speedup_vs_fp64"] = ms64 / ms
         e = ensemble_errors(Ea32, Ea64, DP.E)

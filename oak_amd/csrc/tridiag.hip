@@ -732,6 +732,16 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
         continue;
       }
       if (++iter > 60) { st = 2; break; }
+      if constexpr (NP == 256) {
+        // k_tql_apply_rows reads the log a window of 256 records at a time: a
+        // chain that would cross into the next window starts there, after a
+        // header with m = 0 that closes the window
+        if ((pos & 255) + 1 + (m - l) > 256) {
+          if (pos + 1 > cap) { st = 1; break; }
+          lg[rlog_rec(pos)] = make_double2(0.0, 0.0);
+          pos = (pos | 255) + 1;
+        }
+      }
       if (pos + 1 + (m - l) > cap) { st = 1; break; }
       const double el = e[l][lane], dl = d[l][lane];
       double g = (d[l + 1][lane] - dl) / (2.0 * el);
@@ -908,8 +918,11 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0, con
 // U = Q Z for NP = 256: the rows of Q are independent, so workgroup y applies
 // the logged rotations to rows RB y .. RB y + RB - 1 in LDS (NP = 256 rows
 // would need 512 KB), one thread per row, with the operations of
-// k_tql_apply.  Each thread also stages NP / RB records of the next chain.
-// A zone whose QL did not finish is reported (no fallback at this size).
+// k_tql_apply.  The log is staged a window of NP records at a time (no chain
+// crosses a window, see k_tql), each thread loading NP / RB records of the
+// next window while the current one is applied: one wait per window rather
+// than per chain.  A zone whose QL did not finish is reported (no fallback
+// at this size).
 // ---------------------------------------------------------------------------
 template <int NP, int RB>
 __global__ __launch_bounds__(RB) void k_tql_apply_rows(int64_t z0, const int32_t* __restrict__ zorder,
@@ -940,31 +953,35 @@ __global__ __launch_bounds__(RB) void k_tql_apply_rows(int64_t z0, const int32_t
                    [&](int e, double x) { Z[(e / NP) * LD + e % NP] = x; });
   const int64_t len = stz >> 2;
   const double2* lg = rlog + rlog_off(zl, cap);
+  // the log in windows of NP records (k_tql starts no chain that would cross
+  // one): window w + 1 is loaded while the chains of window w are applied
 #pragma unroll
   for (int u = 0; u < NQ; ++u) {
     const int q = tid + RB * u;
     if (q < len) buf[0][q] = lg[rlog_rec(q)];
   }
   __syncthreads();
-  int64_t pos = 0;
   int b = 0;
-  while (pos < len) {
-    const double2 h = buf[b][0];
-    const int m = __builtin_amdgcn_readfirstlane(((int)h.x) >> 16);
-    const int cnt = __builtin_amdgcn_readfirstlane((int)h.y);
-    const int64_t nxt = pos + 1 + cnt;
+  for (int64_t w0 = 0; w0 < len; w0 += NP) {
+    const int64_t w1 = w0 + NP;
     double2 pv[NQ];
 #pragma unroll
     for (int u = 0; u < NQ; ++u) {
       const int q = tid + RB * u;
-      pv[u] = nxt + q < len ? lg[rlog_rec(nxt + q)] : make_double2(0.0, 0.0);
+      pv[u] = w1 + q < len ? lg[rlog_rec(w1 + q)] : make_double2(0.0, 0.0);
     }
-    {
+    const int wl = (int)(len - w0 < NP ? len - w0 : NP);
+    int pos = 0;
+    while (pos < wl) {
+      const double2 h = buf[b][pos];
+      const int m = __builtin_amdgcn_readfirstlane(((int)h.x) >> 16);
+      if (m == 0) break;  // the rest of the window holds no chain
+      const int cnt = __builtin_amdgcn_readfirstlane((int)h.y);
       // column i of this row is p[0]; rotations i = m-1, m-2, ... with
       // (c, s) = CS(0), CS(1), ... (the loop of k_tql_apply)
       typedef __attribute__((address_space(3))) const double lds_d;
       double* p = Z + tid * LD + (m - 1);
-      lds_d* cd = (lds_d*)(buf[b] + 1);
+      lds_d* cd = (lds_d*)(buf[b] + pos + 1);
       asm volatile("" : "+v"(cd));  // one VGPR base for the broadcast reads
       double carry = p[1];
       int k = 0;
@@ -1002,14 +1019,14 @@ __global__ __launch_bounds__(RB) void k_tql_apply_rows(int64_t z0, const int32_t
 #undef OAK_ROT
 #undef CS
       p[1] = carry;  // column m - cnt
+      pos += 1 + cnt;
     }
 #pragma unroll
     for (int u = 0; u < NQ; ++u) {
       const int q = tid + RB * u;
-      if (nxt + q < len) buf[b ^ 1][q] = pv[u];
+      if (w1 + q < len) buf[b ^ 1][q] = pv[u];
     }
     __syncthreads();
-    pos = nxt;
     b ^= 1;
   }
   for (int e = tid; e < RB * NP; e += RB) Q[e] = Z[(e / NP) * LD + e % NP];
@@ -1361,15 +1378,15 @@ __global__ __launch_bounds__(256) void k_tql_apply128(int64_t z0, const int32_t*
 // ---------------------------------------------------------------------------
 // Epilogue from eigenvectors U (rows in global memory) and eigenvalues d.
 // ---------------------------------------------------------------------------
-template <int NP>
-__global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
+template <int NP, int BS = 256>
+__global__ __launch_bounds__(BS) void k_eigvec_epi(int64_t z0, const int32_t* __restrict__ zorder, int N, const int64_t* __restrict__ sel_ptr,
                                                     const double* __restrict__ mat,
                                                     double* __restrict__ umat,
                                                     const double* __restrict__ dvec, int nzc,
                                                     double* __restrict__ vec,
                                                     double* __restrict__ ampl,
                                                     unsigned* __restrict__ err) {
-  constexpr int H = 256 / NP;  // threads per column / row
+  constexpr int H = BS / NP;  // threads per column / row
   __shared__ double col[6][NP];
   __shared__ double part[H][NP];
   const int zl = blockIdx.x;
@@ -1444,7 +1461,7 @@ __global__ __launch_bounds__(256) void k_eigvec_epi(int64_t z0, const int32_t* _
   const double vn = vr / sqrt(nv);
   if (hh == 0) col[5][j] = 1.0 / sqrt(sqrt(muc));
   lds_barrier();
-  for (int e = tid; e < NP * NP; e += 256) U[e] = U[e] * col[5][e % NP];
+  for (int e = tid; e < NP * NP; e += BS) U[e] = U[e] * col[5][e % NP];
   if (hh == 0) {
     V[NP + j] = ar;
     V[2 * NP + j] = vn;
@@ -1540,8 +1557,8 @@ void launch_ql256(const ChunkArgs& A, hipStream_t st) {
   hipLaunchKernelGGL((k_tql_apply_rows<NP, 64>), dim3(A.nzc, NP / 64), dim3(64), 0, st, A.z0,
                      A.zorder, A.sel_ptr, A.umat, (const double2*)A.rlog, A.rlog_cap,
                      A.tql_status, A.err);
-  hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
-                     A.mat, A.umat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
+  hipLaunchKernelGGL((k_eigvec_epi<NP, 1024>), dim3(A.nzc), dim3(1024), 0, st, A.z0, A.zorder, A.N,
+                     A.sel_ptr, A.mat, A.umat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
 }
 
 void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st) {

@@ -16,7 +16,7 @@
 //                  scheme of k_formq128 on a 256 x 64 slice), four workgroups
 //                  per zone, one 64-column block of Q each.
 //   tridiag.hip    k_tql (QL on (d, e), rotations logged),
-//                  k_tql_apply_rows (the rotations on 64-row blocks of Q in
+//                  k_tql_apply_rows (the rotations on 32-row blocks of Q in
 //                  LDS) and k_eigvec_epi: eigenvalues, U, a, v and Y.
 //   k_tmat256      T = (Y Y^T) Omega on MFMA, four workgroups per zone, one
 //                  64-row block of T each.
@@ -784,16 +784,24 @@ __global__ __launch_bounds__(256) void k_update256(int nzc, int64_t z0,
     const int64_t row = rb + li;
     const bool rok = row < sz;
     const double xb = (MODE == 1 && rok) ? X0[i0 + row] : 0.0;
-    for (int k0 = 0; k0 < NP; k0 += 4) {
-      const int kc = k0 + kk;
-      double bval = 0.0;
-      if (rok && kc < N) {
-        const double sv = (double)S[(i0 + row) + (int64_t)kc * ldS];
-        bval = (MODE == 1) ? (sv - xb) / scaling : sv;
+    // the members of 8 k-steps are loaded before their MFMAs (a load per
+    // step, waited for at once, would expose the memory latency 64 times)
+    for (int k0 = 0; k0 < NP; k0 += 32) {
+      double bv8[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int kc = k0 + 4 * u + kk;
+        const bool ok = rok && kc < N;
+        const double sv = (double)S[ok ? (i0 + row) + (int64_t)kc * ldS : i0];
+        bv8[u] = ok ? ((MODE == 1) ? (sv - xb) / scaling : sv) : 0.0;
       }
 #pragma unroll
-      for (int J = 0; J < NBc; ++J) acc[J] = mfma_f64(Ws[kc * LW + 16 * J + li], bval, acc[J]);
-      acca = mfma_f64(li == 0 ? as[kc] : 0.0, bval, acca);
+      for (int u = 0; u < 8; ++u) {
+        const int kc = k0 + 4 * u + kk;
+#pragma unroll
+        for (int J = 0; J < NBc; ++J) acc[J] = mfma_f64(Ws[kc * LW + 16 * J + li], bv8[u], acc[J]);
+        acca = mfma_f64(li == 0 ? as[kc] : 0.0, bv8[u], acca);
+      }
     }
     // S a of row (rb + li): column 0 of the a-tile, lanes 0..15, register 0
     const double dx = shfl_d(acca[0], li);

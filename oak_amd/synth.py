@@ -415,6 +415,23 @@ def config2_gpu(N=128, device="cuda:0", dtype=None):
                             dtype=dtype, device=device)
 
 
+def config4_gpu(scale=1, device="cuda:0", dtype=None):
+    """720x360x40 x {T,S,u,v} + SSH (n = 41,731,200 at scale 1), N = 256 FP32
+    members; SSH observations in swaths and 10,000 T profiles of 40 levels
+    (about 1.4M + 0.4M values), localisation L = 3 cells with cutoff 9 (3 L),
+    the swaths drawn so that the local counts are log-normal with median
+    1900 and log standard deviation 0.45, with gaps of no data.  scale > 1
+    divides both horizontal sizes and the number of profiles (scale^2) and
+    keeps the local counts."""
+    import torch
+    nx, ny = 720 // scale, 360 // scale
+    n_prof = max(1, 10000 // (scale * scale))
+    return column_ocean_gpu(nx, ny, 40, 256, 4, n_prof_cols=n_prof, L=3.0, cutoff=9.0,
+                            ssh_density=calibrated_swaths(1900.0, 0.45, 9.0, n_prof=n_prof,
+                                                          nz_lev=40),
+                            dtype=dtype or torch.float32, device=device)
+
+
 def config3(nx=256, N=32, m=4000, seed_cfg=3):
     """Differentiable-analysis configuration: the config 0 layout on an
     nx x nx grid (n = nx^2, one zone per point), N GRF members (l = 8 cells),

@@ -42,7 +42,7 @@ def _close(got, exp, tol):
     np.testing.assert_allclose(got[ok], exp[ok], rtol=tol, atol=0)
 
 
-@pytest.mark.parametrize("N", [1, 7, 16, 33, 64, 100, 128])
+@pytest.mark.parametrize("N", [1, 7, 16, 33, 64, 100, 128, 200, 256])
 def test_report_vs_oracle(A, N):
     m = 5000
     d, HS, sig2, ex = _case(10 + N, m, N)
@@ -87,7 +87,7 @@ def test_report_rejects_bad_arguments(A):
     with pytest.raises(OakError):
         A.report(d, HS, sig2, ex, ranges=[0, 101])
     with pytest.raises(OakError):
-        A.report(d, np.zeros((100, 129), order="F"), sig2, ex)
+        A.report(d, np.zeros((100, 257), order="F"), sig2, ex)  # N above OAKG_MAX_N
 
 
 def test_report_device_pointers_and_leading_dimension(A):

@@ -536,8 +536,11 @@ def main():
     if prec in ("mixed", "both"):
         ctx = Context(0, OAKG_MIXED)
         ctx.set_eigensolver(eig_id)
-        E32 = torch.empty((N, n), dtype=torch.float32, device=device).t()
-        E32.copy_(DP.E)
+        if DP.E.dtype == torch.float32:  # FP32 members already (config 4)
+            E32 = DP.E
+        else:
+            E32 = torch.empty((N, n), dtype=torch.float32, device=device).t()
+            E32.copy_(DP.E)
         Ea32 = torch.empty((N, n), dtype=torch.float32, device=device).t()
         res["mixed"] = run_analysis(args, ctx, g, H, Hshift, yo, rmse2, E32, Ea32, "mixed")
         counts = ctx.counts(P.nzones)

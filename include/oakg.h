@@ -285,7 +285,7 @@ int oakg_ensemble_analysis_f32(oakg_ctx* ctx, const oakg_zones* zones, const oak
  *   stats[5r+2]  bias_yo-Hx                 sum(d, D != 0) / ingrid
  *   stats[5r+3]  rms_invsqrtR_yo-Hx         sqrt(sum(d R^-1 d) / ingrid)
  *   stats[5r+4]  projection_yo-Hx_into_HSf  sqrt(|HS^T R^-1 d|^2 / ingrid)
- * (NaN where ingrid = 0, as in OAK).  HS is m x N (ldHS), N <= 128; every
+ * (NaN where ingrid = 0, as in OAK).  HS is m x N (ldHS), N <= 256; every
  * rmse2 entry > 0.  The sums run in a fixed order: repeated calls give the
  * same bits. */
 int oakg_report(oakg_ctx* ctx, int64_t m, int32_t N, const double* yo_Hx, const double* HS,

@@ -513,9 +513,9 @@ def main():
     g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
     print("# generated %s in %.1f s" % (desc, time.time() - t0), file=sys.stderr, flush=True)
     n, N = DP.E.shape
-    eig_name = (("k_tridiag256+k_formq256+k_tql+k_tql_apply_rows+k_eigvec_epi (Householder on "
+    eig_name = (("k_tridiag256+k_formq256+k_tql+k_tql_apply256+k_eigvec_epi (Householder on "
                  "register tiles, Q in blocked WY form on FP64 MFMA, implicit QL, rotations on "
-                 "64-row blocks, eigenvector epilogue; FP64)" if N > 128 else
+                 "rows in registers, eigenvector epilogue; FP64)" if N > 128 else
                  "k_tridiag (two launches)+k_formq128+k_tql+k_tql_apply128 (Householder, Q in "
                  "blocked WY form on FP64 MFMA, implicit QL, eigenvector epilogue; FP64), with "
                  "the zones of p < 64 through k_tridiag+k_tql+k_tql_apply+k_eigvec_epi at 64 "

@@ -13,8 +13,8 @@ The knobs (oakg_set_tuning) change how the zones are scheduled:
              observation space,
   persist    the Gram kernel as a persistent grid taking zones from a work
              counter (2: a zone per workgroup).
-tri_split, dual and persist act at N > 64 only (``space_for`` leaves them
-out below).  All but dual and tri_split = 2 give the same bits; those give
+tri_split, dual and persist act at 64 < N <= 128 only (``space_for``
+leaves them out elsewhere).  All but dual and tri_split = 2 give the same bits; those give
 the same analysis to rounding.  ``tune`` times every candidate on the caller's own workload
 (device-resident inputs, the same call repeated), keeps the fastest, and
 returns a table entry; ``apply`` sets a context from a table (a JSON file
@@ -39,7 +39,7 @@ def key(N, precision):
 
 def space_for(N):
     """The candidates that change anything at ensemble size N."""
-    if int(N) > 64:
+    if 64 < int(N) <= 128:
         return dict(CANDIDATES)
     return {k: v for k, v in CANDIDATES.items() if k not in WIDE_ONLY}
 

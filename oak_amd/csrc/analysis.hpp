@@ -171,6 +171,7 @@ void launch_tql_pipeline(int NP, const ChunkArgs& A, hipStream_t st);
 // its eigensolver (tridiag.hip)
 void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm);
 void launch_ql256(const ChunkArgs& A, hipStream_t st);
+void launch_tridiag256_tail(const ChunkArgs& A, hipStream_t st);
 void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_nll_adjoint(int NP, const ChunkArgs& A, hipStream_t st);

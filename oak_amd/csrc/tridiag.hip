@@ -106,7 +106,11 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
                                                  const double* __restrict__ mat,
                                                  double* __restrict__ qmat,
                                                  double* __restrict__ dvec, double* __restrict__ evec,
-                                                 int nzc) {
+                                                 int nzc, int64_t zstride, int elast) {
+  // zstride: doubles between the matrices (and the reflector blocks) of
+  // consecutive zones, NP^2 except for the trailing blocks of NP = 256;
+  // elast = 0 leaves e_{NP-1} to the caller (there, the coupling of the two
+  // halves of the NP = 256 tridiagonal)
   // Stage s runs on tiles of Ts over the block [Hs, NP)^2 = 16 Ts rows:
   // T1 = NP/16 over the whole matrix, then T2 = T1/2 and T3 = T2/2 (as far
   // as the tiles go down to 1 x 1).
@@ -125,8 +129,8 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
   const int r1 = tr * T1, c1 = tc * T1;            // tile origins per stage
   const int r2 = H2 + tr * T2, c2 = H2 + tc * T2;
   const int r3 = H3 + tr * T3, c3 = H3 + tc * T3;
-  const double* M = mat + (size_t)zl * NP * NP;
-  double* V = qmat + (size_t)zl * NP * NP;
+  const double* M = mat + (size_t)zl * zstride;
+  double* V = qmat + (size_t)zl * zstride;
 
   // Steps i0 .. i1-1 of the reduction on tiles of T at (r0, c0); tile
   // column tc covers columns base + tc*T ...  Every wave carries the current
@@ -472,7 +476,7 @@ __global__ __launch_bounds__(256) void k_tridiag(int64_t z0, const int32_t* __re
       if (c == NP - 2 && r == NP - 1) evec[(size_t)0 * nzc + zl] = a3[ii][jj];
       if (c == NP - 1 && r == NP - 1) dvec[(size_t)0 * nzc + zl] = a3[ii][jj];
     }
-  if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
+  if (t == 0 && elast) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
   if constexpr (NP == 128) return;  // Q is formed by k_formq128 on MFMA
 
   // ---- Q = H_0 H_1 ... H_{NP-3}, accumulated backwards: Q <- H_k Q ----
@@ -732,16 +736,6 @@ __global__ __launch_bounds__(64) void k_tql(int nzc, int64_t z0, const int32_t* 
         continue;
       }
       if (++iter > 60) { st = 2; break; }
-      if constexpr (NP == 256) {
-        // k_tql_apply_rows reads the log a window of 256 records at a time: a
-        // chain that would cross into the next window starts there, after a
-        // header with m = 0 that closes the window
-        if ((pos & 255) + 1 + (m - l) > 256) {
-          if (pos + 1 > cap) { st = 1; break; }
-          lg[rlog_rec(pos)] = make_double2(0.0, 0.0);
-          pos = (pos | 255) + 1;
-        }
-      }
       if (pos + 1 + (m - l) > cap) { st = 1; break; }
       const double el = e[l][lane], dl = d[l][lane];
       double g = (d[l + 1][lane] - dl) / (2.0 * el);
@@ -915,121 +909,179 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0, con
 }
 
 // ---------------------------------------------------------------------------
-// U = Q Z for NP = 256: the rows of Q are independent, so workgroup y applies
-// the logged rotations to rows RB y .. RB y + RB - 1 in LDS (NP = 256 rows
-// would need 512 KB), one thread per row, with the operations of
-// k_tql_apply.  The log is staged a window of NP records at a time (no chain
-// crosses a window, see k_tql), each thread loading NP / RB records of the
-// next window while the current one is applied: one wait per window rather
-// than per chain.  A zone whose QL did not finish is reported (no fallback
-// at this size).
+// U = Q Z for NP = 256 with the rows in registers, the scheme of
+// k_tql_apply128 (below) at twice the width: workgroup y takes rows
+// 128 y .. 128 y + 127, row r's columns 0..127 in thread r (waves 0-1, the
+// lower group) and columns 128..255 in thread 128 + r (waves 2-3, the upper
+// group), 128 values a thread.  A chain acts on the upper columns
+// (positions >= 128), couples columns 127 and 128 (position 127), then acts
+// on the lower columns; the groups run one chain apart, each in unrolled,
+// branch-free segments of 16 positions with the identity outside the chain.
+// One workgroup per CU (the row halves take 256 registers a lane), every SIMD
+// issuing.
 // ---------------------------------------------------------------------------
-template <int NP, int RB>
-__global__ __launch_bounds__(RB) void k_tql_apply_rows(int64_t z0, const int32_t* __restrict__ zorder,
+struct Apply256Lds {
+  double2 buf[3][256 + 12];  // records of chains t+1, t, t-1
+  double2 pr[4][128];        // per wave: (c, s) of its group's positions
+  double xc[2][2][128];      // [step parity][0: column 127, 1: column 128] per row
+};
+
+__global__ __launch_bounds__(256) void k_tql_apply256(int64_t z0, const int32_t* __restrict__ zorder,
                                                        const int64_t* __restrict__ sel_ptr,
                                                        double* __restrict__ qmat,
                                                        const double2* __restrict__ rlog,
                                                        int64_t cap,
                                                        const int32_t* __restrict__ status,
                                                        unsigned* __restrict__ err) {
-  constexpr int LD = NP + 1;
-  constexpr int PAD = 8;  // room for the look-ahead reads below column 0 of row 0
-  constexpr int NQ = NP / RB;
-  __shared__ double Zs[PAD + RB * LD];
-  __shared__ double2 buf[2][NP + 12];
-  double* Z = Zs + PAD;
+  constexpr int NP = 256, H = 128;
+  __shared__ Apply256Lds S;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
   if (z < 0) return;
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
-  const int stz = status[zl];
   const int tid = threadIdx.x;
+  const int stz = status[zl];
   if (stz & 3) {
     if (blockIdx.y == 0 && tid == 0) atomicOr(err, OAKG_DERR_NOCONV);
     return;
   }
-  double* Q = qmat + (size_t)zl * NP * NP + (size_t)blockIdx.y * RB * NP;
-  copy_batched<16>(tid, RB * NP, RB, [&](int e) { return Q[e]; },
-                   [&](int e, double x) { Z[(e / NP) * LD + e % NP] = x; });
+  const int g = tid >> 7, r = tid & 127, w = tid >> 6, lane = tid & 63;
+  double* Qr = qmat + (size_t)zl * NP * NP + (size_t)(H * blockIdx.y + r) * NP + H * g;
+  double x[H];  // row H y + r, columns H g .. H g + H - 1
+  {
+    const double2* q2 = reinterpret_cast<const double2*>(Qr);
+#pragma unroll
+    for (int c = 0; c < H / 2; ++c) {
+      const double2 v = q2[c];
+      x[2 * c] = v.x;
+      x[2 * c + 1] = v.y;
+    }
+  }
   const int64_t len = stz >> 2;
   const double2* lg = rlog + rlog_off(zl, cap);
-  // the log in windows of NP records (k_tql starts no chain that would cross
-  // one): window w + 1 is loaded while the chains of window w are applied
-#pragma unroll
-  for (int u = 0; u < NQ; ++u) {
-    const int q = tid + RB * u;
-    if (q < len) buf[0][q] = lg[rlog_rec(q)];
-  }
+  if (tid < len) S.buf[0][tid] = lg[rlog_rec(tid)];
   __syncthreads();
-  int b = 0;
-  for (int64_t w0 = 0; w0 < len; w0 += NP) {
-    const int64_t w1 = w0 + NP;
-    double2 pv[NQ];
-#pragma unroll
-    for (int u = 0; u < NQ; ++u) {
-      const int q = tid + RB * u;
-      pv[u] = w1 + q < len ? lg[rlog_rec(w1 + q)] : make_double2(0.0, 0.0);
+  int64_t pos = 0;
+  int bt = 0, bp = 2, par = 0;
+  bool has_prev = false;
+  int m_prev = 0, l_prev = 0;
+  double2* pr = S.pr[w];
+  while (true) {
+    const bool has_cur = pos < len;
+    if (!has_cur && !has_prev) break;
+    int m = 0, l = 0;
+    int64_t nxt = pos;
+    if (has_cur) {
+      const double2 hd = S.buf[bt][0];
+      m = __builtin_amdgcn_readfirstlane(((int)hd.x) >> 16);
+      const int cnt = __builtin_amdgcn_readfirstlane((int)hd.y);
+      l = m - cnt;
+      nxt = pos + 1 + cnt;
     }
-    const int wl = (int)(len - w0 < NP ? len - w0 : NP);
-    int pos = 0;
-    while (pos < wl) {
-      const double2 h = buf[b][pos];
-      const int m = __builtin_amdgcn_readfirstlane(((int)h.x) >> 16);
-      if (m == 0) break;  // the rest of the window holds no chain
-      const int cnt = __builtin_amdgcn_readfirstlane((int)h.y);
-      // column i of this row is p[0]; rotations i = m-1, m-2, ... with
-      // (c, s) = CS(0), CS(1), ... (the loop of k_tql_apply)
-      typedef __attribute__((address_space(3))) const double lds_d;
-      double* p = Z + tid * LD + (m - 1);
-      lds_d* cd = (lds_d*)(buf[b] + pos + 1);
-      asm volatile("" : "+v"(cd));  // one VGPR base for the broadcast reads
-      double carry = p[1];
-      int k = 0;
-#define CS(J) make_double2(cd[2 * (J)], cd[2 * (J) + 1])
-#define OAK_ROT(Q, ZV, DST)                  \
-  DST = fma(Q.y, ZV, Q.x * carry);           \
-  carry = fma(Q.x, ZV, -Q.y * carry);
-      if (cnt >= 8) {
-        double a0 = p[0], a1 = p[-1], a2 = p[-2], a3 = p[-3];
-        double2 qa0 = CS(0), qa1 = CS(1), qa2 = CS(2), qa3 = CS(3);
-        for (; k + 8 <= cnt; k += 8) {
-          const double b0 = p[-4], b1 = p[-5], b2 = p[-6], b3 = p[-7];
-          const double2 qb0 = CS(4), qb1 = CS(5), qb2 = CS(6), qb3 = CS(7);
-          OAK_ROT(qa0, a0, p[1])
-          OAK_ROT(qa1, a1, p[0])
-          OAK_ROT(qa2, a2, p[-1])
-          OAK_ROT(qa3, a3, p[-2])
-          a0 = p[-8]; a1 = p[-9]; a2 = p[-10]; a3 = p[-11];
-          qa0 = CS(8); qa1 = CS(9); qa2 = CS(10); qa3 = CS(11);
-          OAK_ROT(qb0, b0, p[-3])
-          OAK_ROT(qb1, b1, p[-4])
-          OAK_ROT(qb2, b2, p[-5])
-          OAK_ROT(qb3, b3, p[-6])
-          p -= 8;
-          cd += 16;
+    const int bn = bt == 2 ? 0 : bt + 1;
+    const bool pf = has_cur && nxt + tid < len;
+    double2 pv = make_double2(0.0, 0.0);
+    if (pf) pv = lg[rlog_rec(nxt + tid)];
+    // coupling rotation (position 127) of chain t-1
+    if (has_prev && l_prev <= H - 1 && m_prev - 1 >= H - 1) {
+      const double2 q = S.buf[bp][1 + (m_prev - 1) - (H - 1)];
+      const double chi = g == 1 ? x[0] : S.xc[par ^ 1][1][r];
+      const double clo = g == 0 ? x[H - 1] : S.xc[par ^ 1][0][r];
+      const double nhi = fma(q.y, clo, q.x * chi);
+      const double nlo = fma(q.x, clo, -q.y * chi);
+      if (g == 1) x[0] = nhi;
+      else x[H - 1] = nlo;
+    }
+    if (g == 1) {
+      if (has_cur && m - 1 >= H) {
+        // chain t on columns 128..255: positions 254..128 in eight unrolled
+        // segments of 16, the identity (1, 0) outside l..m-1
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int P = H + 64 * u + lane;
+          pr[64 * u + lane] =
+              (P >= l && P <= m - 1) ? S.buf[bt][1 + (m - 1 - P)] : make_double2(1.0, 0.0);
         }
-      }
-      for (; k < cnt; ++k) {
-        const double zv = p[0];
-        const double2 q = CS(0);
-        OAK_ROT(q, zv, p[1])
-        p -= 1;
-        cd += 2;
-      }
-#undef OAK_ROT
-#undef CS
-      p[1] = carry;  // column m - cnt
-      pos += 1 + cnt;
-    }
+        auto segment = [&](auto top) {
+          constexpr int T = decltype(top)::value;
+          if (m - 1 < T - 15 || l > T) return;
+          double2 qv[16];
 #pragma unroll
-    for (int u = 0; u < NQ; ++u) {
-      const int q = tid + RB * u;
-      if (w1 + q < len) buf[b ^ 1][q] = pv[u];
+          for (int t = 0; t < 16; ++t)
+            if (T - t >= H) qv[t] = pr[T - t - H];
+#pragma unroll
+          for (int t = 0; t < 16; ++t) {
+            const int J = T - t;
+            if (J >= H) {
+              const double hh = x[J + 1 - H], zj = x[J - H];
+              x[J + 1 - H] = fma(qv[t].y, zj, qv[t].x * hh);
+              x[J - H] = fma(qv[t].x, zj, -qv[t].y * hh);
+            }
+          }
+        };
+        segment(std::integral_constant<int, 254>{});
+        segment(std::integral_constant<int, 238>{});
+        segment(std::integral_constant<int, 222>{});
+        segment(std::integral_constant<int, 206>{});
+        segment(std::integral_constant<int, 190>{});
+        segment(std::integral_constant<int, 174>{});
+        segment(std::integral_constant<int, 158>{});
+        segment(std::integral_constant<int, 142>{});
+      }
+      S.xc[par][1][r] = x[0];
+    } else {
+      if (has_prev && l_prev <= H - 2) {
+        // chain t-1 on columns 0..127: positions 126..l
+        const int mp = m_prev, lp = l_prev;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int P = 64 * u + lane;
+          pr[P] = (P >= lp && P <= mp - 1 && P <= H - 2) ? S.buf[bp][1 + (mp - 1 - P)]
+                                                         : make_double2(1.0, 0.0);
+        }
+        auto segment_lo = [&](auto top) {
+          constexpr int T = decltype(top)::value;
+          if (mp - 1 < T - 15 || lp > T) return;
+          double2 qv[16];
+#pragma unroll
+          for (int t = 0; t < 16; ++t)
+            if (T - t >= 0) qv[t] = pr[T - t];
+#pragma unroll
+          for (int t = 0; t < 16; ++t) {
+            const int J = T - t;
+            if (J >= 0) {
+              const double hh = x[J + 1], zj = x[J];
+              x[J + 1] = fma(qv[t].y, zj, qv[t].x * hh);
+              x[J] = fma(qv[t].x, zj, -qv[t].y * hh);
+            }
+          }
+        };
+        segment_lo(std::integral_constant<int, 126>{});
+        segment_lo(std::integral_constant<int, 110>{});
+        segment_lo(std::integral_constant<int, 94>{});
+        segment_lo(std::integral_constant<int, 78>{});
+        segment_lo(std::integral_constant<int, 62>{});
+        segment_lo(std::integral_constant<int, 46>{});
+        segment_lo(std::integral_constant<int, 30>{});
+        segment_lo(std::integral_constant<int, 14>{});
+      }
+      S.xc[par][0][r] = x[H - 1];
     }
+    if (pf) S.buf[bn][tid] = pv;
     __syncthreads();
-    b ^= 1;
+    has_prev = has_cur;
+    m_prev = m;
+    l_prev = l;
+    bp = bt;
+    bt = bn;
+    pos = nxt;
+    par ^= 1;
   }
-  for (int e = tid; e < RB * NP; e += RB) Q[e] = Z[(e / NP) * LD + e % NP];
+  {
+    double2* q2 = reinterpret_cast<double2*>(Qr);
+#pragma unroll
+    for (int c = 0; c < H / 2; ++c) q2[c] = make_double2(x[2 * c], x[2 * c + 1]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1478,19 +1530,19 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   if constexpr (NP == 128) split = A.tri_split;
   if (split == 2) {
     hipLaunchKernelGGL((k_tridiag<128, 3>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
     hipLaunchKernelGGL((k_tridiag<128, 4>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
     hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
   } else if (split == 1) {
     hipLaunchKernelGGL((k_tridiag<128, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
     hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
   } else {
     hipLaunchKernelGGL((k_tridiag<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
-                       A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
   }
   if constexpr (NP == 128)
     hipLaunchKernelGGL(k_formq128, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr, A.ymat);
@@ -1530,8 +1582,24 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
   }
 }
 
-// NP = 256: QL on the chunk's (d, e), the rotations on Q (umat) in 64-row
-// blocks, then the epilogue (a, v, mu and Y in umat).
+// NP = 256, steps 128..253 of the Householder reduction: the trailing
+// 128 x 128 block that k_tridiag256 leaves in rows 128..191 of each zone's
+// reflector block, reduced by the NP = 128 kernels in two launches (their
+// reflectors into rows 192..255, zone stride 256^2).  Their d and e land
+// in entries 0..127 and 0..126 of the reversed tridiagonal, as the steps of
+// one 256 reduction would write them.
+void launch_tridiag256_tail(const ChunkArgs& A, hipStream_t st) {
+  constexpr int64_t ZS = 256 * 256;
+  double* blk = A.ymat + 128 * 256;
+  double* v2 = A.ymat + 192 * 256;
+  hipLaunchKernelGGL((k_tridiag<128, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                     blk, v2, A.dvec, A.evec, A.nzc, ZS, 0);
+  hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                     blk, v2, A.dvec, A.evec, A.nzc, ZS, 0);
+}
+
+// NP = 256: QL on the chunk's (d, e), the rotations on Q (umat) with the
+// rows in registers, then the epilogue (a, v, mu and Y in umat).
 void launch_ql256(const ChunkArgs& A, hipStream_t st) {
   constexpr int NP = 256;
 #define OAK_TQL256(ZW)                                                                                    \
@@ -1554,9 +1622,8 @@ void launch_ql256(const ChunkArgs& A, hipStream_t st) {
   else
     OAK_TQL256(16);
 #undef OAK_TQL256
-  hipLaunchKernelGGL((k_tql_apply_rows<NP, 64>), dim3(A.nzc, NP / 64), dim3(64), 0, st, A.z0,
-                     A.zorder, A.sel_ptr, A.umat, (const double2*)A.rlog, A.rlog_cap,
-                     A.tql_status, A.err);
+  hipLaunchKernelGGL(k_tql_apply256, dim3(A.nzc, 2), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                     A.umat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.err);
   hipLaunchKernelGGL((k_eigvec_epi<NP, 1024>), dim3(A.nzc), dim3(1024), 0, st, A.z0, A.zorder, A.N,
                      A.sel_ptr, A.mat, A.umat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
 }

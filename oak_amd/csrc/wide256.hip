@@ -8,20 +8,21 @@
 //                  one or two 64 x 64 blocks of the lower triangle
 //                  (v_mfma_f64_16x16x4f64).
 //   k_tridiag256   Householder reduction to tridiagonal form (lower, as
-//                  dsytd2) with the lower triangle in 8 x 8 register tiles
-//                  over 512 threads, one workgroup per zone: 496 tiles hold
-//                  columns 8..255, and columns 0..7 sit in LDS for the seven
-//                  steps that change them.
+//                  dsytd2), steps 0..127, with the lower triangle in 8 x 8
+//                  register tiles over 512 threads, one workgroup per zone:
+//                  496 tiles hold columns 8..255, and columns 0..7 sit in
+//                  LDS for the seven steps that change them; steps 128..253
+//                  on the trailing 128 x 128 block by k_tridiag<128>.
 //   k_formq256     Q = H_0 ... H_253 in blocked compact-WY form on MFMA (the
 //                  scheme of k_formq128 on a 256 x 64 slice), four workgroups
 //                  per zone, one 64-column block of Q each.
-//   tridiag.hip    k_tql (QL on (d, e), rotations logged),
-//                  k_tql_apply_rows (the rotations on 32-row blocks of Q in
-//                  LDS) and k_eigvec_epi: eigenvalues, U, a, v and Y.
+//   tridiag.hip    k_tql (QL on (d, e), rotations logged), k_tql_apply256
+//                  (the rotations on the rows of Q in registers, two threads
+//                  a row) and k_eigvec_epi: eigenvalues, U, a, v and Y.
 //   k_tmat256      T = (Y Y^T) Omega on MFMA, four workgroups per zone, one
 //                  64-row block of T each.
-//   k_update256    out = S [T | a] for the zone's rows, four workgroups per
-//                  zone (one 64-column block of T each) on one XCD.
+//   k_update256    out = S [T | a] for the zone's rows, eight workgroups per
+//                  zone (one 32-column block of T each) on one XCD.
 // The operations are those of the N <= 128 path (rrsqrt.F90:100-190 and the
 // post-processing of assimilation.F90:3299-3363); only the split of the work
 // and the order of the partial sums differ.
@@ -204,7 +205,10 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
   }
 }
 
-// grid (zones, 2): workgroup y holds wave slots 4 y .. 4 y + 3
+// grid (zones, 2): workgroup y holds wave slots 4 y .. 4 y + 3.  (Giving each
+// XCD a contiguous run of zones, so that neighbours share their rows of S in
+// one L2, was slower: the densely observed zones are contiguous too, and
+// their XCDs finish last.)
 template <typename TS>
 __global__ __launch_bounds__(256) void k_gram256(int64_t z0, const int32_t* __restrict__ zorder,
                                                  const int64_t* __restrict__ sel_ptr,
@@ -263,8 +267,11 @@ __global__ __launch_bounds__(256) void k_gram256(int64_t z0, const int32_t* __re
 // tiles then apply A -= v w^T + w v^T with w = p - (tau/2)(v^T p) v.  Column
 // i+1 as it stands before the update is published by its tile column (or
 // the LDS block), and every wave forms its updated value itself.  Two
-// barriers per step.  Output as k_tridiag: reflector i in row i of qmat
-// (tau_i in entry i), T reversed (d'_k = d_{255-k}).
+// barriers per step.  Steps 0..127 run here; the trailing 128 x 128 block
+// then goes to the NP = 128 kernels (fewer registers, two to four zones per
+// CU through the late, latency-bound steps; launch_tridiag256_tail).
+// Output as k_tridiag: reflector i in row i of qmat (tau_i in entry i), T
+// reversed (d'_k = d_{255-k}).
 // ---------------------------------------------------------------------------
 struct Tri256Lds {
   static constexpr int PR = 288;  // padded length of a row vector (tri_rx(255) = 286)
@@ -283,7 +290,7 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
                                                     double* __restrict__ qmat,
                                                     double* __restrict__ dvec,
                                                     double* __restrict__ evec, int nzc) {
-  constexpr int NP = NPW, NH = 4, PR = Tri256Lds::PR;
+  constexpr int NP = NPW, NH = 4, PR = Tri256Lds::PR, H = NP / 2;
   __shared__ Tri256Lds S;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
   __syncthreads();
   auto pick = [&](int h) { return h == 0 ? x[0] : (h == 1 ? x[1] : (h == 2 ? x[2] : x[3])); };
   double* col = S.colw[wave];
-  for (int i = 0; i < NP - 2; ++i) {
+  for (int i = 0; i < H; ++i) {
     const int b = i & 1;
     double* pw = S.pw[b];
     double* pub = S.pub[b];
@@ -494,12 +501,21 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
       }
     }
   }
-  if (t == 0) {  // tile (31, 31): the trailing 2 x 2 block, rows and columns 254..255
-    dvec[(size_t)1 * nzc + zl] = a[6][6];
-    evec[(size_t)0 * nzc + zl] = a[7][6];
-    dvec[(size_t)0 * nzc + zl] = a[7][7];
-    evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
+  // the trailing block [H, NP)^2 after step H - 1 (the tiles of tile rows and
+  // columns >= 16; both triangles) to rows H .. H + H/2 - 1 of the zone's
+  // reflector block, as a dense H x H matrix: the NP = 128 kernels reduce it
+  // (launch_tridiag256_tail)
+  if (own && tc >= H / 8) {
+    double* blk = V + (size_t)H * NP;
+#pragma unroll
+    for (int ii = 0; ii < 8; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        blk[(size_t)(r0 - H + ii) * H + (c0 - H + jj)] = a[ii][jj];
+        if (tr != tc) blk[(size_t)(c0 - H + jj) * H + (r0 - H + ii)] = a[ii][jj];
+      }
   }
+  if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -543,12 +559,20 @@ __global__ __launch_bounds__(256) void k_formq256(int64_t z0, const int32_t* __r
   // thread t moves reflector pj = t / 16 of a block, rows pr + 16 e (e < 16)
   const int pj = t >> 4, pr = t & 15;
   double pf[16];
+  // reflectors 0..127 are rows of the zone's block; 128..253 (from the NP =
+  // 128 kernels) are rows 0..125 of the 128 x 128 block at row 192, for
+  // rows 128..255 of Q
+  const double* V2 = V + (size_t)192 * NP;
   auto load_block = [&](int b) {
     const int k = 16 * b + pj;
-    if (k < NP - 2) {
+    if (k < NP / 2) {
       const double* src = V + (size_t)k * NP + pr;
 #pragma unroll
       for (int e = 0; e < 16; ++e) pf[e] = src[16 * e];
+    } else if (k < NP - 2) {
+      const double* src = V2 + (size_t)(k - NP / 2) * (NP / 2) + pr;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) pf[e] = e < 8 ? 0.0 : src[16 * (e - 8)];
     } else {
 #pragma unroll
       for (int e = 0; e < 16; ++e) pf[e] = 0.0;
@@ -666,10 +690,9 @@ __global__ __launch_bounds__(256) void k_tmat256(int64_t z0, const int32_t* __re
   for (int J = 0; J < 16; ++J) acc[J] = d4{0, 0, 0, 0};
   for (int k0 = 0; k0 < NP; k0 += KC) {
     __syncthreads();
-    for (int e = threadIdx.x; e < NP * KC; e += 256) {
-      const int r = e / KC, c = e % KC;
-      Yc[r * LY + c] = Yz[(size_t)r * NP + k0 + c];
-    }
+    copy_batched<16>(threadIdx.x, NP * KC, 256,
+                     [&](int e) { return Yz[(size_t)(e / KC) * NP + k0 + e % KC]; },
+                     [&](int e, double y) { Yc[(e / KC) * LY + e % KC] = y; });
     __syncthreads();
 #pragma unroll
     for (int k4 = 0; k4 < KC; k4 += 4) {
@@ -732,11 +755,11 @@ __global__ __launch_bounds__(256) void k_tmat256(int64_t z0, const int32_t* __re
 // ---------------------------------------------------------------------------
 // Update: out = S_z [T | a] for the zone's rows, transposed on the MFMA as in
 // k_update (A = [T|a]^T from LDS, B = S_z^T from global).  Workgroup g:
-// zone 8 (g / 32) + g % 8 of the chunk, columns 64 cb .. 64 cb + 63 of T with
-// cb = (g / 8) % 4, so the four workgroups of a zone land on one XCD (the
+// zone 8 (g / 64) + g % 8 of the chunk, columns 32 cb .. 32 cb + 31 of T with
+// cb = (g / 8) % 8, so the eight workgroups of a zone land on one XCD (the
 // dispatcher deals workgroups to the XCDs in turn) and share its L2 for the
-// zone's rows of S.  Each workgroup forms S a itself (bit-identical in all
-// four), for the mean and the clamp.
+// zone's rows of S; 68 KB of LDS each, two per CU.  Each workgroup forms
+// S a itself (bit-identical in all eight), for the mean and the clamp.
 //   MODE 0: xa = xf + Sf a, Sa = Sf T            (rrsqrt.F90:151,185,463)
 //   MODE 1: Sf = (E - xbar)/sqrt(N-1) on the fly; xa = xbar + clamp(Sf a);
 //           Ea = xa + sqrt(N-1) * (inflation * Sf T)  (assimilation.F90:3301-3320)
@@ -755,11 +778,11 @@ __global__ __launch_bounds__(256) void k_update256(int nzc, int64_t z0,
                                                    double maxcorr,
                                                    const double* __restrict__ maxcorr_v) {
 #pragma clang fp contract(off)
-  constexpr int NP = NPW, CB = 64, NBc = CB / 16, LW = CB + 1;
+  constexpr int NP = NPW, CB = 32, NCB = NP / CB, NBc = CB / 16, LW = CB + 1;
   __shared__ double Ws[NP * LW];
   __shared__ double as[NP];
   const int g = blockIdx.x;
-  const int zl = (g >> 5) * 8 + (g & 7), cb = (g >> 3) & 3;
+  const int zl = (g / (8 * NCB)) * 8 + (g & 7), cb = (g >> 3) % NCB;
   if (zl >= nzc) return;
   const int64_t z = zone_at(zorder, z0 + zl);
   if (z < 0) return;
@@ -768,10 +791,9 @@ __global__ __launch_bounds__(256) void k_update256(int nzc, int64_t z0,
   if (sz <= 0) return;
   const double* T = mat + (size_t)zl * NP * NP;
   const double* a = vec + (size_t)zl * 4 * NP + NP;
-  for (int e = threadIdx.x; e < NP * CB; e += 256) {
-    const int k = e / CB, c = e % CB;
-    Ws[k * LW + c] = T[(size_t)k * NP + CB * cb + c];
-  }
+  copy_batched<16>(threadIdx.x, NP * CB, 256,
+                   [&](int e) { return T[(size_t)(e / CB) * NP + CB * cb + e % CB]; },
+                   [&](int e, double t) { Ws[(e / CB) * LW + e % CB] = t; });
   as[threadIdx.x] = a[threadIdx.x];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -848,6 +870,7 @@ void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_EIG);
   hipLaunchKernelGGL(k_tridiag256, dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder, A.sel_ptr, A.mat,
                      A.ymat, A.dvec, A.evec, A.nzc);
+  launch_tridiag256_tail(A, st);
   hipLaunchKernelGGL(k_formq256, dim3(A.nzc, 4), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                      A.ymat, A.umat);
   launch_ql256(A, st);
@@ -856,7 +879,7 @@ void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   hipLaunchKernelGGL(k_tmat256, dim3(A.nzc, 4), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                      A.umat, A.mat, A.vec);
   if (tm) tm->mark(PH_UPDATE);
-  const unsigned nb = (unsigned)(((A.nzc + 7) / 8) * 32);
+  const unsigned nb = (unsigned)(((A.nzc + 7) / 8) * 64);  // 8 column blocks per zone
   if (A.prec == 1)
     hipLaunchKernelGGL((k_update256<1, float>), dim3(nb), dim3(256), 0, st, A.nzc, A.z0, A.zorder,
                        A.N, A.zone_ptr, A.mat, A.vec, A.X0, A.S32, A.ldS, A.xa, A.Out32, A.ldO,

@@ -11,6 +11,8 @@ def test_space_leaves_out_the_wide_kernel_knobs_at_64():
     assert set(s128) == set(autotune.CANDIDATES)
     assert not set(autotune.WIDE_ONLY) & set(s64)
     assert set(s64) | set(autotune.WIDE_ONLY) == set(s128)
+    # N > 128 has its own kernels: the N = 128 options do nothing there
+    assert set(autotune.space_for(256)) == set(s64)
     # every default is a candidate, and every candidate is a value the
     # library takes (oakg_set_tuning ranges)
     ok = {"slots": range(1, 5), "ql_zones": (4, 8, 16, 32), "graph": (0, 1), "order": (0, 1),

@@ -3,9 +3,9 @@
 // A zone's 256 x 256 matrix is 512 KB in FP64: it fits neither the
 // registers of one workgroup nor its LDS, where the N <= 128 kernels keep
 // a zone whole, so each phase splits it its own way:
-//   k_gram256      M = I + S~^T S~ and b = S~^T R^-1 d: eight waves per zone
-//                  (two workgroups), each over all local observations for
-//                  one or two 64 x 64 blocks of the lower triangle
+//   k_gram256      M = I + S~^T S~ and b = S~^T R^-1 d: 16 waves per zone
+//                  (four workgroups), each over all local observations for
+//                  two or three 32 x 32 blocks of the lower triangle
 //                  (v_mfma_f64_16x16x4f64).
 //   k_tridiag256   Householder reduction to tridiagonal form (lower, as
 //                  dsytd2), steps 0..127, with the lower triangle in 8 x 8
@@ -37,28 +37,33 @@ constexpr int NPW = 256;
 }
 
 // ---------------------------------------------------------------------------
-// Gram.  Wave slot ws (0..7) of a zone: slots 0..5 hold the off-diagonal
-// 64 x 64 blocks (X, Y), X > Y, of the lower triangle (16 tiles of 16 x 16),
-// slots 6 and 7 two diagonal blocks X and Y each (2 x 10 tiles).  Every wave
-// runs over all the zone's observations, four per MFMA step, with the next
-// step's gathers in flight (the scheme of k_gram).  Slots 6 and 7 also form
-// b for the columns of their blocks: together they cover all 256.
+// Gram.  The lower triangle of M in 32 x 32 blocks (2 x 2 tiles of 16 x 16):
+// 28 off-diagonal blocks (4 tiles) and 8 diagonal ones (3 tiles), dealt to
+// 16 wave slots per zone (four workgroups): slots 0..11 two off-diagonal
+// blocks each (8 tiles), slots 12..15 one off-diagonal and two diagonal
+// blocks (10 tiles), which also form b for the columns of their diagonal
+// blocks (all 256 together).  Every wave runs over all the zone's
+// observations, four per MFMA step, with the next step's gathers in flight
+// (the scheme of k_gram); at about 200 registers two waves share a SIMD, so
+// one wave's gathers and VALU work overlap another's MFMAs.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void gram256_slot(int ws, int& X, int& Y) {
-  switch (ws) {
-    case 0: X = 1; Y = 0; break;
-    case 1: X = 2; Y = 0; break;
-    case 2: X = 3; Y = 0; break;
-    case 3: X = 2; Y = 1; break;
-    case 4: X = 3; Y = 1; break;
-    case 5: X = 3; Y = 2; break;
-    case 6: X = 0; Y = 3; break;
-    default: X = 1; Y = 2; break;
+// off-diagonal block k (0..27) of the 8 x 8 block grid: (bi, bj), bi > bj
+__device__ __forceinline__ void gram256_offdiag(int k, int& bi, int& bj) {
+  int i = 1;
+  while (k >= i) {
+    k -= i;
+    ++i;
   }
+  bi = i;
+  bj = k;
 }
 
-template <typename TS, bool DIAG>
-__device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p,
+// MIX = false: off-diagonal blocks (X0, Y0) and (X1, Y1); MIX = true:
+// off-diagonal block (X0, Y0) and diagonal blocks X1 and Y1.  The lane's
+// values u[4 h + 2 s + k] = row[32 B + 16 k + jl] with B = (h ? (s ? Y1 : X1)
+// : (s ? Y0 : X0)).
+template <typename TS, bool MIX>
+__device__ __forceinline__ void gram256_wave(int X0, int Y0, int X1, int Y1, int64_t p0, int64_t p,
                                              const int32_t* __restrict__ sel_idx,
                                              const double* __restrict__ sel_w, int64_t m,
                                              const double* __restrict__ rho,
@@ -67,14 +72,14 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
                                              double* __restrict__ bv_out,
                                              unsigned* __restrict__ err) {
   constexpr int NP = NPW;
-  constexpr int NT = DIAG ? 20 : 16;
+  constexpr int NT = MIX ? 10 : 8;
   const int lane = threadIdx.x & 63, kq = lane >> 4, jl = lane & 15;
   d4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = d4{0, 0, 0, 0};
-  double bacc[8];
+  double bacc[4];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) bacc[k] = 0.0;
+  for (int k = 0; k < 4; ++k) bacc[k] = 0.0;
   const int64_t nsteps = (p + 3) >> 2;
   struct Idx {
     int64_t l;
@@ -91,7 +96,7 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
     x.w = sel_w[p0 + qc];
     return x;
   };
-  // u[0..3]: columns 64 X + 16 k + jl, u[4..7]: columns 64 Y + 16 k + jl
+  const int c0 = 32 * X0 + jl, c1 = 32 * Y0 + jl, c2 = 32 * X1 + jl, c3 = 32 * Y1 + jl;
   auto fetch_row = [&](Idx& x, double& rh, double& inn, TS* rv) {
     const bool inr = x.l >= 0 && x.l < m;
     bad = bad | (x.ok & !inr);
@@ -99,11 +104,13 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
     x.l = x.ok ? x.l : 0;
     rh = rho[x.l];
     inn = innov[x.l];
-    const TS* row = HS + (size_t)x.l * NP + jl;
+    const TS* row = HS + (size_t)x.l * NP;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      rv[k] = row[64 * X + 16 * k];
-      rv[4 + k] = row[64 * Y + 16 * k];
+    for (int k = 0; k < 2; ++k) {
+      rv[k] = row[c0 + 16 * k];
+      rv[2 + k] = row[c1 + 16 * k];
+      rv[4 + k] = row[c2 + 16 * k];
+      rv[6 + k] = row[c3 + 16 * k];
     }
   };
   Idx i0 = fetch_idx(0), i1 = fetch_idx(1);
@@ -123,27 +130,36 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
 #pragma unroll
     for (int k = 0; k < 8; ++k) bv[k] = ok ? (double)rv0[k] : 0.0;
     int t = 0;
-    if constexpr (DIAG) {
+#pragma unroll
+    for (int I = 0; I < 2; ++I) {  // off-diagonal block (X0, Y0)
+      const double a = r * bv[I];
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        acc[t] = mfma_f64(a, bv[2 + J], acc[t]);
+        ++t;
+      }
+    }
+    if constexpr (MIX) {  // diagonal blocks X1 and Y1
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int I = 0; I < 4; ++I) {
-          const double a = r * bv[4 * h + I];
+        for (int I = 0; I < 2; ++I) {
+          const double a = r * bv[4 + 2 * h + I];
 #pragma unroll
           for (int J = 0; J <= I; ++J) {
-            acc[t] = mfma_f64(a, bv[4 * h + J], acc[t]);
+            acc[t] = mfma_f64(a, bv[4 + 2 * h + J], acc[t]);
             ++t;
           }
         }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) bacc[k] = fma(rd, bv[k], bacc[k]);
-    } else {
+      for (int k = 0; k < 4; ++k) bacc[k] = fma(rd, bv[4 + k], bacc[k]);
+    } else {  // off-diagonal block (X1, Y1)
 #pragma unroll
-      for (int I = 0; I < 4; ++I) {
-        const double a = r * bv[I];
+      for (int I = 0; I < 2; ++I) {
+        const double a = r * bv[4 + I];
 #pragma unroll
-        for (int J = 0; J < 4; ++J) {
-          acc[t] = mfma_f64(a, bv[4 + J], acc[t]);
+        for (int J = 0; J < 2; ++J) {
+          acc[t] = mfma_f64(a, bv[6 + J], acc[t]);
           ++t;
         }
       }
@@ -156,15 +172,15 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
     for (int k = 0; k < 8; ++k) rv0[k] = rv1[k];
   }
   if (bad) atomicOr(err, OAKG_DERR_IDX);
-  if constexpr (DIAG) {
+  if constexpr (MIX) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 4; ++k) {
       bacc[k] += shfl_xor_d(bacc[k], 16);
       bacc[k] += shfl_xor_d(bacc[k], 32);
     }
     if (lane < 16) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) bv_out[64 * (k < 4 ? X : Y) + 16 * (k & 3) + lane] = bacc[k];
+      for (int k = 0; k < 4; ++k) bv_out[32 * (k < 2 ? X1 : Y1) + 16 * (k & 1) + lane] = bacc[k];
     }
   }
   // tile (gi0, gj0): register q of lane l is C[l/16 + 4q][l%16]; a diagonal
@@ -183,42 +199,50 @@ __device__ __forceinline__ void gram256_wave(int X, int Y, int64_t p0, int64_t p
     }
   };
   int t = 0;
-  if constexpr (DIAG) {
+#pragma unroll
+  for (int I = 0; I < 2; ++I)
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      put(32 * X0 + 16 * I, 32 * Y0 + 16 * J, acc[t], false);
+      ++t;
+    }
+  if constexpr (MIX) {
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int I = 0; I < 4; ++I)
+      for (int I = 0; I < 2; ++I)
 #pragma unroll
         for (int J = 0; J <= I; ++J) {
-          const int B = h ? Y : X;
-          put(64 * B + 16 * I, 64 * B + 16 * J, acc[t], I == J);
+          const int B = h ? Y1 : X1;
+          put(32 * B + 16 * I, 32 * B + 16 * J, acc[t], I == J);
           ++t;
         }
   } else {
 #pragma unroll
-    for (int I = 0; I < 4; ++I)
+    for (int I = 0; I < 2; ++I)
 #pragma unroll
-      for (int J = 0; J < 4; ++J) {
-        put(64 * X + 16 * I, 64 * Y + 16 * J, acc[t], false);
+      for (int J = 0; J < 2; ++J) {
+        put(32 * X1 + 16 * I, 32 * Y1 + 16 * J, acc[t], false);
         ++t;
       }
   }
 }
 
-// grid (zones, 2): workgroup y holds wave slots 4 y .. 4 y + 3.  (Giving each
+// grid (zones, 4): workgroup y holds wave slots 4 y .. 4 y + 3.  (Giving each
 // XCD a contiguous run of zones, so that neighbours share their rows of S in
 // one L2, was slower: the densely observed zones are contiguous too, and
 // their XCDs finish last.)
 template <typename TS>
-__global__ __launch_bounds__(256) void k_gram256(int64_t z0, const int32_t* __restrict__ zorder,
-                                                 const int64_t* __restrict__ sel_ptr,
-                                                 const int32_t* __restrict__ sel_idx,
-                                                 const double* __restrict__ sel_w, int64_t m,
-                                                 const double* __restrict__ rho,
-                                                 const double* __restrict__ innov,
-                                                 const TS* __restrict__ HS,
-                                                 double* __restrict__ mat, double* __restrict__ vec,
-                                                 unsigned* __restrict__ err) {
+__global__ __launch_bounds__(256, 2) void k_gram256(int64_t z0, const int32_t* __restrict__ zorder,
+                                                    const int64_t* __restrict__ sel_ptr,
+                                                    const int32_t* __restrict__ sel_idx,
+                                                    const double* __restrict__ sel_w, int64_t m,
+                                                    const double* __restrict__ rho,
+                                                    const double* __restrict__ innov,
+                                                    const TS* __restrict__ HS,
+                                                    double* __restrict__ mat,
+                                                    double* __restrict__ vec,
+                                                    unsigned* __restrict__ err) {
   constexpr int NP = NPW;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
@@ -227,22 +251,29 @@ __global__ __launch_bounds__(256) void k_gram256(int64_t z0, const int32_t* __re
   const int64_t p = sel_ptr[z + 1] - p0;
   double* M = mat + (size_t)zl * NP * NP;
   double* bv_out = vec + (size_t)zl * 4 * NP;
-  if (p == 0) {  // M = I (rows 128 y .. 128 y + 127), b = 0
+  if (p == 0) {  // M = I (rows 64 y .. 64 y + 63), b = 0
     const int y = blockIdx.y;
-    for (int e = threadIdx.x; e < NP * NP / 2; e += 256) {
-      const int g = y * (NP * NP / 2) + e;
+    for (int e = threadIdx.x; e < NP * NP / 4; e += 256) {
+      const int g = y * (NP * NP / 4) + e;
       M[g] = (g / NP == g % NP) ? 1.0 : 0.0;
     }
     if (y == 0) bv_out[threadIdx.x] = 0.0;
     return;
   }
   const int ws = __builtin_amdgcn_readfirstlane((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6));
-  int X, Y;
-  gram256_slot(ws, X, Y);
-  if (ws >= 6)
-    gram256_wave<TS, true>(X, Y, p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bv_out, err);
-  else
-    gram256_wave<TS, false>(X, Y, p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bv_out, err);
+  int X0, Y0, X1, Y1;
+  if (ws < 12) {
+    gram256_offdiag(2 * ws, X0, Y0);
+    gram256_offdiag(2 * ws + 1, X1, Y1);
+    gram256_wave<TS, false>(X0, Y0, X1, Y1, p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bv_out,
+                            err);
+  } else {
+    gram256_offdiag(24 + (ws - 12), X0, Y0);
+    X1 = 2 * (ws - 12);
+    Y1 = X1 + 1;
+    gram256_wave<TS, true>(X0, Y0, X1, Y1, p0, p, sel_idx, sel_w, m, rho, innov, HS, M, bv_out,
+                           err);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -860,11 +891,11 @@ __global__ __launch_bounds__(256) void k_update256(int nzc, int64_t z0,
 void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
   if (tm) tm->mark(PH_GRAM);
   if (A.prec == 1)
-    hipLaunchKernelGGL((k_gram256<float>), dim3(A.nzc, 2), dim3(256), 0, st, A.z0, A.zorder,
+    hipLaunchKernelGGL((k_gram256<float>), dim3(A.nzc, 4), dim3(256), 0, st, A.z0, A.zorder,
                        A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS32, A.mat, A.vec,
                        A.err);
   else
-    hipLaunchKernelGGL((k_gram256<double>), dim3(A.nzc, 2), dim3(256), 0, st, A.z0, A.zorder,
+    hipLaunchKernelGGL((k_gram256<double>), dim3(A.nzc, 4), dim3(256), 0, st, A.z0, A.zorder,
                        A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec,
                        A.err);
   if (tm) tm->mark(PH_EIG);

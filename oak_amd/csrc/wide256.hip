@@ -789,7 +789,8 @@ __global__ __launch_bounds__(256) void k_tmat256(int64_t z0, const int32_t* __re
 // zone 8 (g / 64) + g % 8 of the chunk, columns 32 cb .. 32 cb + 31 of T with
 // cb = (g / 8) % 8, so the eight workgroups of a zone land on one XCD (the
 // dispatcher deals workgroups to the XCDs in turn) and share its L2 for the
-// zone's rows of S; 68 KB of LDS each, two per CU.  Each workgroup forms
+// zone's rows of S; 68 KB of LDS each, two per CU.  The members of the next
+// 8 k-steps are loaded while those of the current 8 are multiplied.  Each workgroup forms
 // S a itself (bit-identical in all eight), for the mean and the clamp.
 //   MODE 0: xa = xf + Sf a, Sa = Sf T            (rrsqrt.F90:151,185,463)
 //   MODE 1: Sf = (E - xbar)/sqrt(N-1) on the fly; xa = xbar + clamp(Sf a);
@@ -837,24 +838,35 @@ __global__ __launch_bounds__(256) void k_update256(int nzc, int64_t z0,
     const int64_t row = rb + li;
     const bool rok = row < sz;
     const double xb = (MODE == 1 && rok) ? X0[i0 + row] : 0.0;
-    // the members of 8 k-steps are loaded before their MFMAs (a load per
-    // step, waited for at once, would expose the memory latency 64 times)
-    for (int k0 = 0; k0 < NP; k0 += 32) {
-      double bv8[8];
+    // the members of 8 k-steps at a time, the next 8 loaded while the
+    // current ones are multiplied (a load per step, waited for at once,
+    // would expose the memory latency 64 times)
+    auto load8 = [&](int k0, TE* raw) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int kc = k0 + 4 * u + kk;
         const bool ok = rok && kc < N;
-        const double sv = (double)S[ok ? (i0 + row) + (int64_t)kc * ldS : i0];
-        bv8[u] = ok ? ((MODE == 1) ? (sv - xb) / scaling : sv) : 0.0;
+        raw[u] = S[ok ? (i0 + row) + (int64_t)kc * ldS : i0];
       }
+    };
+    TE cur[8];
+    load8(0, cur);
+    for (int k0 = 0; k0 < NP; k0 += 32) {
+      TE nxt[8];
+      load8(k0 + 32 < NP ? k0 + 32 : k0, nxt);
+      asm volatile("" ::: "memory");  // compiler barrier: keeps the loads above here
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int kc = k0 + 4 * u + kk;
+        const bool ok = rok && kc < N;
+        const double sv = (double)cur[u];
+        const double bval = ok ? ((MODE == 1) ? (sv - xb) / scaling : sv) : 0.0;
 #pragma unroll
-        for (int J = 0; J < NBc; ++J) acc[J] = mfma_f64(Ws[kc * LW + 16 * J + li], bv8[u], acc[J]);
-        acca = mfma_f64(li == 0 ? as[kc] : 0.0, bv8[u], acca);
+        for (int J = 0; J < NBc; ++J) acc[J] = mfma_f64(Ws[kc * LW + 16 * J + li], bval, acc[J]);
+        acca = mfma_f64(li == 0 ? as[kc] : 0.0, bval, acca);
       }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
     }
     // S a of row (rb + li): column 0 of the a-tile, lanes 0..15, register 0
     const double dx = shfl_d(acca[0], li);

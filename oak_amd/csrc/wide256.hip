@@ -688,8 +688,9 @@ __global__ __launch_bounds__(256) void k_formq256(int64_t z0, const int32_t* __r
 // T = (Y Y^T) Omega (the operator of k_tmat: Omega = RotateVector(w, v) as
 // the identity plus a rank-structured term), rows 64 y .. 64 y + 63 in
 // workgroup y: wave w forms tile row I = 4 y + w of Y Y^T (16 tiles in its
-// accumulators, Y staged through LDS 16 columns at a time), then its rows'
-// products with the vectors of Omega, then T.  Zones without observations
+// accumulators, Y staged through LDS 16 columns at a time, the next block
+// loaded while the current one is multiplied), then its rows' products with
+// the vectors of Omega, then T.  Zones without observations
 // get T = I.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tmat256(int64_t z0, const int32_t* __restrict__ zorder,
@@ -698,7 +699,7 @@ __global__ __launch_bounds__(256) void k_tmat256(int64_t z0, const int32_t* __re
                                                  double* __restrict__ mat,
                                                  const double* __restrict__ vec) {
   constexpr int NP = NPW, KC = 16, LY = KC + 1;
-  __shared__ double Yc[NP * LY];
+  __shared__ double Yc[2][NP * LY];
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
   if (z < 0) return;
@@ -719,18 +720,34 @@ __global__ __launch_bounds__(256) void k_tmat256(int64_t z0, const int32_t* __re
   d4 acc[16];
 #pragma unroll
   for (int J = 0; J < 16; ++J) acc[J] = d4{0, 0, 0, 0};
+  // columns k0 .. k0 + 15 of Y in LDS buffer (k0 / 16) % 2; thread e holds
+  // 16 values of the next block (rows e / 16 + 16 u, column e % 16) in
+  // registers while the current block is multiplied
+  double yn[16];
+  auto load_y = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) yn[u] = Yz[(size_t)(threadIdx.x / KC + 16 * u) * NP + k0 + threadIdx.x % KC];
+  };
+  auto store_y = [&](int b) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Yc[b][(threadIdx.x / KC + 16 * u) * LY + threadIdx.x % KC] = yn[u];
+  };
+  load_y(0);
+  store_y(0);
+  __syncthreads();
   for (int k0 = 0; k0 < NP; k0 += KC) {
-    __syncthreads();
-    copy_batched<16>(threadIdx.x, NP * KC, 256,
-                     [&](int e) { return Yz[(size_t)(e / KC) * NP + k0 + e % KC]; },
-                     [&](int e, double y) { Yc[(e / KC) * LY + e % KC] = y; });
-    __syncthreads();
+    const int b = (k0 / KC) & 1;
+    if (k0 + KC < NP) load_y(k0 + KC);
+    asm volatile("" ::: "memory");  // compiler barrier: keeps the loads above here
+    const double* Yb = Yc[b];
 #pragma unroll
     for (int k4 = 0; k4 < KC; k4 += 4) {
-      const double av = Yc[(16 * I + li) * LY + k4 + kk];
+      const double av = Yb[(16 * I + li) * LY + k4 + kk];
 #pragma unroll
-      for (int J = 0; J < 16; ++J) acc[J] = mfma_f64(av, Yc[(16 * J + li) * LY + k4 + kk], acc[J]);
+      for (int J = 0; J < 16; ++J) acc[J] = mfma_f64(av, Yb[(16 * J + li) * LY + k4 + kk], acc[J]);
     }
+    if (k0 + KC < NP) store_y(b ^ 1);
+    __syncthreads();
   }
   // register q of lane l: row 16 I + l/16 + 4 q, column 16 J + l%16
   const double wN = 1.0 / sqrt((double)N);

@@ -1535,6 +1535,11 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
     hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
+  } else if (split == 3) {
+    // steps 0..63 on the lower triangle only (wide256.hip), then as split 1
+    launch_tridiag_lower128(A, st);
+    hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
+                       A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
   } else if (split == 1) {
     hipLaunchKernelGGL((k_tridiag<128, 1>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);

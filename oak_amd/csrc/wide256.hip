@@ -304,37 +304,42 @@ __global__ __launch_bounds__(256, 2) void k_gram256(int64_t z0, const int32_t* _
 // Output as k_tridiag: reflector i in row i of qmat (tau_i in entry i), T
 // reversed (d'_k = d_{255-k}).
 // ---------------------------------------------------------------------------
-struct Tri256Lds {
-  static constexpr int PR = 288;  // padded length of a row vector (tri_rx(255) = 286)
-  double colw[8][PR];   // each wave's copy of v
-  double pub[2][PR];    // column i+1 before step i's update (double-buffered)
-  double pw[2][PR];     // p (double-buffered)
-  double part[32][PR];  // slot s of row r: its partial sum of A v
-  double l0[256][9];    // columns 0..7 of every row (steps 0..6)
-  double hp[4][8];      // the row waves' shares of (A v)_0..7 (steps 0..6)
-  double red[2][4];     // the row waves' shares of v^T p (double-buffered)
+template <int NP>
+struct TriLowerLds {
+  static constexpr int PR = (tri_rx(NP - 1) + 2) & ~1;  // padded length of a row vector
+  static constexpr int NW = NP == 256 ? 8 : 2;         // waves of the workgroup
+  static constexpr int NRW = NP / 64;                  // row waves
+  double colw[NW][PR];     // each wave's copy of v
+  double pub[2][PR];       // column i+1 before step i's update (double-buffered)
+  double pw[2][PR];        // p (double-buffered)
+  double part[NP / 8][PR]; // slot s of row r: its partial sum of A v
+  double l0[NP][9];        // columns 0..7 of every row (steps 0..6)
+  double hp[NRW][8];       // the row waves' shares of (A v)_0..7 (steps 0..6)
+  double red[2][NRW];      // the row waves' shares of v^T p (double-buffered)
 };
 
-__global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* __restrict__ zorder,
-                                                    const int64_t* __restrict__ sel_ptr,
-                                                    const double* __restrict__ mat,
-                                                    double* __restrict__ qmat,
-                                                    double* __restrict__ dvec,
-                                                    double* __restrict__ evec, int nzc) {
-  constexpr int NP = NPW, NH = 4, PR = Tri256Lds::PR, H = NP / 2;
-  __shared__ Tri256Lds S;
+template <int NP>
+__global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
+    int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
+    const double* __restrict__ mat, double* __restrict__ qmat, double* __restrict__ dvec,
+    double* __restrict__ evec, int nzc) {
+  using L = TriLowerLds<NP>;
+  constexpr int NH = NP / 64, PR = L::PR, H = NP / 2, NB = NP / 8;
+  constexpr int NT = (NB - 1) * NB / 2;  // register tiles (tile columns 1..NB-1)
+  constexpr int BS = 64 * L::NW;         // threads
+  __shared__ L S;
   const int zl = blockIdx.x;
   const int64_t z = zone_at(zorder, z0 + zl);
   if (z < 0) return;
   if (sel_ptr[z + 1] == sel_ptr[z]) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const bool own = t < 496;
-  int tr = 31, tc = 31;
-  if (own) {  // ring s = 31 - tc holds the s + 1 tiles of tile column tc
+  const bool own = t < NT;
+  int tr = NB - 1, tc = NB - 1;
+  if (own) {  // ring s = NB - 1 - tc holds the s + 1 tiles of tile column tc
     int s = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
     while (s * (s + 1) / 2 > t) --s;
     while ((s + 1) * (s + 2) / 2 <= t) ++s;
-    tc = 31 - s;
+    tc = NB - 1 - s;
     tr = tc + (t - s * (s + 1) / 2);
   }
   const int r0 = 8 * tr, c0 = 8 * tc;
@@ -346,17 +351,22 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
   for (int ii = 0; ii < 8; ++ii)
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) a[ii][jj] = own ? M[(size_t)(r0 + ii) * NP + c0 + jj] : 0.0;
-  const int row = t - 256;  // waves 4..7: row threads
+  const int row = t - (BS - NP);  // the last NP threads: row threads
   if (row >= 0) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) S.l0[row][c] = M[(size_t)row * NP + c];
   }
-  for (int e = t; e < 32 * PR; e += 512) (&S.part[0][0])[e] = 0.0;
+  for (int e = t; e < NB * PR; e += BS) (&S.part[0][0])[e] = 0.0;
   double x[NH];
 #pragma unroll
   for (int h = 0; h < NH; ++h) x[h] = M[(size_t)(lane + 64 * h) * NP];  // column 0 (M symmetric)
   __syncthreads();
-  auto pick = [&](int h) { return h == 0 ? x[0] : (h == 1 ? x[1] : (h == 2 ? x[2] : x[3])); };
+  auto pick = [&](int h) {
+    double v = x[0];
+#pragma unroll
+    for (int k = 1; k < NH; ++k) v = h == k ? x[k] : v;
+    return v;
+  };
   double* col = S.colw[wave];
   for (int i = 0; i < H; ++i) {
     const int b = i & 1;
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
         col[tri_rx(r)] = v[h];
       }
       __builtin_amdgcn_wave_barrier();
-      if (wave == 7) {
+      if (wave == L::NW - 1) {
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
           const int r = lane + 64 * h;
@@ -472,7 +482,7 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
         for (int c = 0; c < 8; ++c) hs[c] = wave_sum_d(hs[c]);
         if (lane == 0) {
 #pragma unroll
-          for (int c = 0; c < 8; ++c) S.hp[wave - 4][c] = hs[c];
+          for (int c = 0; c < 8; ++c) S.hp[row >> 6][c] = hs[c];
         }
       } else if (i == 7) {
         S.part[0][tri_rx(row)] = 0.0;  // the block has left the trailing matrix
@@ -486,18 +496,28 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
 #pragma unroll
       for (int q = 0; q < 4; ++q) s4[q] = S.part[q][rowx];
 #pragma unroll
-      for (int q = 4; q < 32; ++q) s4[q & 3] += S.part[q][rowx];
+      for (int q = 4; q < NB; ++q) s4[q & 3] += S.part[q][rowx];
       double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-      if (i < 7 && row < 8) s = (S.hp[0][row] + S.hp[1][row]) + (S.hp[2][row] + S.hp[3][row]);
+      if (i < 7 && row < 8) {
+        if constexpr (NP == 256)
+          s = (S.hp[0][row] + S.hp[1][row]) + (S.hp[2][row] + S.hp[3][row]);
+        else
+          s = S.hp[0][row] + S.hp[1][row];
+      }
       const double pr = row > i ? tau * s : 0.0;
       pw[rowx] = pr;
       double vp = col[rowx] * pr;
       vp = wave_sum_d(vp);
-      if (lane == 0) S.red[b][wave - 4] = vp;
+      if (lane == 0) S.red[b][row >> 6] = vp;
     }
     lds_barrier();
     // A -= v w^T + w v^T, w = p + K v, K = -tau/2 v^T p
-    const double vpt = (S.red[b][0] + S.red[b][1]) + (S.red[b][2] + S.red[b][3]);
+    double vpt;
+    if constexpr (NP == 256) {
+      vpt = (S.red[b][0] + S.red[b][1]) + (S.red[b][2] + S.red[b][3]);
+    } else {
+      vpt = S.red[b][0] + S.red[b][1];
+    }
     const double K = -0.5 * tau * vpt;
     if (act) {
       double wC[8];
@@ -533,21 +553,32 @@ __global__ __launch_bounds__(512) void k_tridiag256(int64_t z0, const int32_t* _
     }
   }
   // the trailing block [H, NP)^2 after step H - 1 (the tiles of tile rows and
-  // columns >= 16; both triangles) to rows H .. H + H/2 - 1 of the zone's
-  // reflector block, as a dense H x H matrix: the NP = 128 kernels reduce it
-  // (launch_tridiag256_tail)
+  // columns >= H / 8; both triangles) to the rows from H of the zone's
+  // reflector block: NP = 256, as a dense H x H matrix that the NP = 128
+  // kernels reduce (launch_tridiag256_tail); NP = 128, as row r, column
+  // c - H of rows H.. (the layout k_tridiag<128, 1> leaves for
+  // k_tridiag<128, 2>)
+  constexpr int LDO = NP == 256 ? H : NP;
   if (own && tc >= H / 8) {
     double* blk = V + (size_t)H * NP;
 #pragma unroll
     for (int ii = 0; ii < 8; ++ii)
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        blk[(size_t)(r0 - H + ii) * H + (c0 - H + jj)] = a[ii][jj];
-        if (tr != tc) blk[(size_t)(c0 - H + jj) * H + (r0 - H + ii)] = a[ii][jj];
+        blk[(size_t)(r0 - H + ii) * LDO + (c0 - H + jj)] = a[ii][jj];
+        if (tr != tc) blk[(size_t)(c0 - H + jj) * LDO + (r0 - H + ii)] = a[ii][jj];
       }
   }
-  if (t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
+  if (NP == 256 && t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
 }
+
+// NP = 128, tri_split 3: steps 0..63 on the lower-triangle tiles (two waves,
+// four zones per CU), then k_tridiag<128, 2> from the trailing block
+void launch_tridiag_lower128(const ChunkArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_tridiag_lower<128>, dim3(A.nzc), dim3(128), 0, st, A.z0, A.zorder,
+                     A.sel_ptr, A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+}
+
 
 // ---------------------------------------------------------------------------
 // Q = H_0 H_1 ... H_253 from the reflectors in the rows of qmat (row k: v_k,
@@ -928,8 +959,8 @@ void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
                        A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec,
                        A.err);
   if (tm) tm->mark(PH_EIG);
-  hipLaunchKernelGGL(k_tridiag256, dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder, A.sel_ptr, A.mat,
-                     A.ymat, A.dvec, A.evec, A.nzc);
+  hipLaunchKernelGGL(k_tridiag_lower<256>, dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder, A.sel_ptr,
+                     A.mat, A.ymat, A.dvec, A.evec, A.nzc);
   launch_tridiag256_tail(A, st);
   hipLaunchKernelGGL(k_formq256, dim3(A.nzc, 4), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                      A.ymat, A.umat);

@@ -155,18 +155,21 @@ def test_dual_excluded_observations_and_single_observation_zones():
     ctx.close()
 
 
+@pytest.mark.parametrize("split", [2, 3])
 @pytest.mark.parametrize("N", [100, 128])
-def test_three_launch_reduction_vs_oracle(N):
+def test_three_launch_reduction_vs_oracle(N, split):
     """OAKG_TUNE_TRIDIAG_SPLIT = 2 (steps 32..63 of the Householder reduction
-    on 6 x 6 tiles: the partial sums are grouped differently) against the
-    oracle, with and without the observation-space bucket."""
+    on 6 x 6 tiles) and 3 (steps 0..63 on the lower triangle only, the
+    upper part of A v from column sums): the partial sums are grouped
+    differently; against the oracle, with and without the observation-space
+    bucket."""
     from oak_amd import assimilation as A
     P = _problem(N, 600 + N)
     H = A.coo_to_csr(P.m, P.Hi, P.Hj, P.Hs)
     Eo, _, _ = _oracle(P, 1.05, 0.1)
     xf = P.E.mean(axis=1)
     for dual in (0, 64):
-        ctx = _ctx(tri_split=2, dual=dual, order=1)
+        ctx = _ctx(tri_split=split, dual=dual, order=1)
         Ea = A.ensemble_analysis(_geo(P), H, P.Hshift, P.yo, P.rmse2, P.E, inflation=1.05,
                                  max_correction=0.1, ctx=ctx)
         assert rel_err(Ea - xf[:, None], Eo - xf[:, None]) < TOL64, (N, dual)

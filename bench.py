@@ -513,7 +513,7 @@ def main():
     g, H, Hshift, yo, rmse2 = device_inputs(DP, device)
     print("# generated %s in %.1f s" % (desc, time.time() - t0), file=sys.stderr, flush=True)
     n, N = DP.E.shape
-    eig_name = (("k_tridiag256+k_formq256+k_tql+k_tql_apply256+k_eigvec_epi (Householder on "
+    eig_name = (("k_tridiag_lower<256>+k_tridiag<128>+k_formq256+k_tql+k_tql_apply256+k_eigvec_epi (Householder on "
                  "register tiles, Q in blocked WY form on FP64 MFMA, implicit QL, rotations on "
                  "rows in registers, eigenvector epilogue; FP64)" if N > 128 else
                  "k_tridiag (two launches)+k_formq128+k_tql+k_tql_apply128 (Householder, Q in "

@@ -1588,7 +1588,7 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
 }
 
 // NP = 256, steps 128..253 of the Householder reduction: the trailing
-// 128 x 128 block that k_tridiag256 leaves in rows 128..191 of each zone's
+// 128 x 128 block that k_tridiag_lower<256> leaves in rows 128..191 of each zone's
 // reflector block, reduced by the NP = 128 kernels in two launches (their
 // reflectors into rows 192..255, zone stride 256^2).  Their d and e land
 // in entries 0..127 and 0..126 of the reversed tridiagonal, as the steps of

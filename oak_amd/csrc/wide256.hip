@@ -7,7 +7,7 @@
 //                  (four workgroups), each over all local observations for
 //                  two or three 32 x 32 blocks of the lower triangle
 //                  (v_mfma_f64_16x16x4f64).
-//   k_tridiag256   Householder reduction to tridiagonal form (lower, as
+//   k_tridiag_lower<256>  Householder reduction to tridiagonal form (lower, as
 //                  dsytd2), steps 0..127, with the lower triangle in 8 x 8
 //                  register tiles over 512 threads, one workgroup per zone:
 //                  496 tiles hold columns 8..255, and columns 0..7 sit in
@@ -25,7 +25,9 @@
 //                  zone (one 32-column block of T each) on one XCD.
 // The operations are those of the N <= 128 path (rrsqrt.F90:100-190 and the
 // post-processing of assimilation.F90:3299-3363); only the split of the work
-// and the order of the partial sums differ.
+// and the order of the partial sums differ.  k_tridiag_lower<128> (the same
+// lower-triangle reduction at NP = 128, steps 0..63) also serves the
+// N = 128 path as its reduction option 3 (launch_tridiag_lower128).
 #include "analysis.hpp"
 #include "common.hpp"
 #include "eigen_common.hpp"

@@ -148,7 +148,7 @@ int oakg_set_timing(oakg_ctx* ctx, int on);
 int oakg_set_eigensolver(oakg_ctx* ctx, int method);
 int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
 /* Tuning knobs of the zone loop (each zone is computed alone, so every
- * setting but OAKG_TUNE_DUAL and OAKG_TUNE_TRIDIAG_SPLIT = 2, 3 gives the same
+ * setting but OAKG_TUNE_DUAL and OAKG_TUNE_TRIDIAG_SPLIT = 2, 3, 4 gives the same
  * bits; those two give the same analysis to rounding):
  *   OAKG_TUNE_SLOTS       zone chunks in flight on as many streams (1..4,
  *                         default 3; from 4096 zones, phase timing off)
@@ -170,8 +170,9 @@ int oakg_get_stats(oakg_ctx* ctx, oakg_stats* out);
  *                         on 6 x 6 register tiles (sums grouped differently);
  *                         3: two kernels, the first with the lower triangle
  *                         only (half the registers per zone, four zones per
- *                         CU; sums grouped differently); 0: one kernel
- *                         (default)
+ *                         CU; sums grouped differently); 4: as 3, the second
+ *                         on the lower triangle as well (one wave per zone);
+ *                         0: one kernel (default)
  *   OAKG_TUNE_DUAL        at N > 64, zones with 1 <= p < value local
  *                         observations (value 16, 32 or 64; 0 = off, the
  *                         default) are solved in the p x p observation space:

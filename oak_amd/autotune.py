@@ -9,13 +9,13 @@ The knobs (oakg_set_tuning) change how the zones are scheduled:
   tri_split  the N > 64 Householder reduction as two kernels (the second
              with fewer registers and more zones per CU), three (the
              middle one on 6 x 6 register tiles), or two with the first on
-             the lower triangle only (3),
+             the lower triangle only (3), or both on it (4),
   dual       the bucket bound below which a zone is solved in its p x p
              observation space,
   persist    the Gram kernel as a persistent grid taking zones from a work
              counter (2: a zone per workgroup).
 tri_split, dual and persist act at 64 < N <= 128 only (``space_for``
-leaves them out elsewhere).  All but dual and tri_split = 2, 3 give the same bits; those
+leaves them out elsewhere).  All but dual and tri_split = 2, 3, 4 give the same bits; those
 give the same analysis to rounding.  ``tune`` times every candidate on the caller's own workload
 (device-resident inputs, the same call repeated), keeps the fastest, and
 returns a table entry; ``apply`` sets a context from a table (a JSON file
@@ -26,7 +26,7 @@ import os
 import time
 
 CANDIDATES = {"slots": (2, 3, 4), "ql_zones": (4, 8, 16), "graph": (0, 1), "order": (0, 1),
-              "tri_split": (0, 1, 2, 3), "dual": (0, 32, 64), "persist": (0, 2)}
+              "tri_split": (0, 1, 2, 3, 4), "dual": (0, 32, 64), "persist": (0, 2)}
 DEFAULT = {"slots": 3, "ql_zones": 16, "graph": 0, "order": 0, "tri_split": 0, "dual": 0,
            "persist": 0}
 WIDE_ONLY = ("tri_split", "dual", "persist")  # knobs of the N > 64 kernels

@@ -1669,7 +1669,7 @@ int oakg_set_tuning(oakg_ctx* c, int key, int value) {
       c->tune_order = value;
       return OAKG_OK;
     case OAKG_TUNE_TRIDIAG_SPLIT:
-      if (value < 0 || value > 3) break;
+      if (value < 0 || value > 4) break;
       c->tune_tri_split = value;
       return OAKG_OK;
     case OAKG_TUNE_DUAL:

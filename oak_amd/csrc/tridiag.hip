@@ -585,7 +585,7 @@ __global__ __launch_bounds__(256, 2) void k_formq128(int64_t z0, const int32_t* 
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int r = pr + 16 * e;
-        vb[r * LV + pj] = r == k ? 0.0 : pf[e];
+        vb[r * LV + pj] = r <= k ? 0.0 : pf[e];  // v_k is zero above row k + 1
         if (r == k) S.tau[buf][pj] = pf[e];
       }
     }
@@ -1535,6 +1535,11 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
     hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
+  } else if (split == 4) {
+    // steps 0..63 on the lower triangle only, then the trailing block by the
+    // lower-triangle kernel at NP = 64, one wave per zone (wide256.hip)
+    launch_tridiag_lower128(A, st);
+    launch_tridiag_lower128_tail(A, st);
   } else if (split == 3) {
     // steps 0..63 on the lower triangle only (wide256.hip), then as split 1
     launch_tridiag_lower128(A, st);

@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256, 2) void k_gram256(int64_t z0, const int32_t* _
 template <int NP>
 struct TriLowerLds {
   static constexpr int PR = (tri_rx(NP - 1) + 2) & ~1;  // padded length of a row vector
-  static constexpr int NW = NP == 256 ? 8 : 2;         // waves of the workgroup
+  static constexpr int NW = NP == 256 ? 8 : (NP == 128 ? 2 : 1);  // waves of the workgroup
   static constexpr int NRW = NP / 64;                  // row waves
   double colw[NW][PR];     // each wave's copy of v
   double pub[2][PR];       // column i+1 before step i's update (double-buffered)
@@ -320,11 +320,18 @@ struct TriLowerLds {
   double red[2][NRW];      // the row waves' shares of v^T p (double-buffered)
 };
 
-template <int NP>
-__global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
+// FULL = false: steps 0 .. NP/2 - 1, then the trailing block is handed on
+// (below); FULL = true: every step, and the trailing 2 x 2 block gives the
+// last two d and the last e (e_{NP-1} is left to the caller).  zstride, ldm
+// and ldv: the distance between the zones' matrices and the leading
+// dimensions of M and of the reflector rows (NP for a zone's own block; the
+// N = 128 trailing block, reduced by k_tridiag_lower<64, true>, sits in rows
+// 64.. of the zone's NP = 128 block).
+template <int NP, bool FULL = false>
+__global__ __launch_bounds__(NP == 256 ? 512 : (NP == 128 ? 128 : 64)) void k_tridiag_lower(
     int64_t z0, const int32_t* __restrict__ zorder, const int64_t* __restrict__ sel_ptr,
     const double* __restrict__ mat, double* __restrict__ qmat, double* __restrict__ dvec,
-    double* __restrict__ evec, int nzc) {
+    double* __restrict__ evec, int nzc, int64_t zstride, int ldm, int ldv) {
   using L = TriLowerLds<NP>;
   constexpr int NH = NP / 64, PR = L::PR, H = NP / 2, NB = NP / 8;
   constexpr int NT = (NB - 1) * NB / 2;  // register tiles (tile columns 1..NB-1)
@@ -346,22 +353,22 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
   }
   const int r0 = 8 * tr, c0 = 8 * tc;
   const int R0 = tri_rx(r0), C0 = tri_rx(c0);  // tri_rx(r0 + ii) = R0 + ii
-  const double* M = mat + (size_t)zl * NP * NP;
-  double* V = qmat + (size_t)zl * NP * NP;
+  const double* M = mat + (size_t)zl * zstride;
+  double* V = qmat + (size_t)zl * zstride;
   double a[8][8];
 #pragma unroll
   for (int ii = 0; ii < 8; ++ii)
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) a[ii][jj] = own ? M[(size_t)(r0 + ii) * NP + c0 + jj] : 0.0;
+    for (int jj = 0; jj < 8; ++jj) a[ii][jj] = own ? M[(size_t)(r0 + ii) * ldm + c0 + jj] : 0.0;
   const int row = t - (BS - NP);  // the last NP threads: row threads
   if (row >= 0) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S.l0[row][c] = M[(size_t)row * NP + c];
+    for (int c = 0; c < 8; ++c) S.l0[row][c] = M[(size_t)row * ldm + c];
   }
   for (int e = t; e < NB * PR; e += BS) (&S.part[0][0])[e] = 0.0;
   double x[NH];
 #pragma unroll
-  for (int h = 0; h < NH; ++h) x[h] = M[(size_t)(lane + 64 * h) * NP];  // column 0 (M symmetric)
+  for (int h = 0; h < NH; ++h) x[h] = M[(size_t)(lane + 64 * h) * ldm];  // column 0 (M symmetric)
   __syncthreads();
   auto pick = [&](int h) {
     double v = x[0];
@@ -370,7 +377,7 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
     return v;
   };
   double* col = S.colw[wave];
-  for (int i = 0; i < H; ++i) {
+  for (int i = 0; i < (FULL ? NP - 2 : H); ++i) {
     const int b = i & 1;
     double* pw = S.pw[b];
     double* pub = S.pub[b];
@@ -419,7 +426,7 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
           const int r = lane + 64 * h;
-          V[(size_t)i * NP + r] = r == i ? tau : v[h];
+          V[(size_t)i * ldv + r] = r == i ? tau : v[h];
         }
         if (lane == 0) {
           dvec[(size_t)(NP - 1 - i) * nzc + zl] = di;
@@ -503,8 +510,10 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
       if (i < 7 && row < 8) {
         if constexpr (NP == 256)
           s = (S.hp[0][row] + S.hp[1][row]) + (S.hp[2][row] + S.hp[3][row]);
-        else
+        else if constexpr (NP == 128)
           s = S.hp[0][row] + S.hp[1][row];
+        else
+          s = S.hp[0][row];
       }
       const double pr = row > i ? tau * s : 0.0;
       pw[rowx] = pr;
@@ -517,8 +526,10 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
     double vpt;
     if constexpr (NP == 256) {
       vpt = (S.red[b][0] + S.red[b][1]) + (S.red[b][2] + S.red[b][3]);
-    } else {
+    } else if constexpr (NP == 128) {
       vpt = S.red[b][0] + S.red[b][1];
+    } else {
+      vpt = S.red[b][0];
     }
     const double K = -0.5 * tau * vpt;
     if (act) {
@@ -560,6 +571,14 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
   // kernels reduce (launch_tridiag256_tail); NP = 128, as row r, column
   // c - H of rows H.. (the layout k_tridiag<128, 1> leaves for
   // k_tridiag<128, 2>)
+  if constexpr (FULL) {
+    if (t == 0) {  // tile (NB-1, NB-1): the trailing 2 x 2 block, rows NP-2, NP-1
+      dvec[(size_t)1 * nzc + zl] = a[6][6];
+      evec[(size_t)0 * nzc + zl] = a[7][6];
+      dvec[(size_t)0 * nzc + zl] = a[7][7];
+    }
+    return;
+  }
   constexpr int LDO = NP == 256 ? H : NP;
   if (own && tc >= H / 8) {
     double* blk = V + (size_t)H * NP;
@@ -574,11 +593,32 @@ __global__ __launch_bounds__(NP == 256 ? 512 : 128) void k_tridiag_lower(
   if (NP == 256 && t == 0) evec[(size_t)(NP - 1) * nzc + zl] = 0.0;
 }
 
+// e_{NP-1} = 0 of every zone of a chunk (NP = 128)
+__global__ void k_fill_elast128(double* __restrict__ evec, int nzc) {
+  const int zl = blockIdx.x * 256 + threadIdx.x;
+  if (zl < nzc) evec[(size_t)127 * nzc + zl] = 0.0;
+}
+
 // NP = 128, tri_split 3: steps 0..63 on the lower-triangle tiles (two waves,
 // four zones per CU), then k_tridiag<128, 2> from the trailing block
 void launch_tridiag_lower128(const ChunkArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_tridiag_lower<128>, dim3(A.nzc), dim3(128), 0, st, A.z0, A.zorder,
-                     A.sel_ptr, A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+  hipLaunchKernelGGL((k_tridiag_lower<128>), dim3(A.nzc), dim3(128), 0, st, A.z0, A.zorder,
+                     A.sel_ptr, A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)128 * 128, 128, 128);
+}
+
+// tri_split 4: the trailing 64 x 64 block (rows 64.., columns 0..63 of the
+// zone's reflector block) reduced in full by k_tridiag_lower<64> (one wave
+// per zone, eight zones per CU): its reflectors go to columns 64..127 of the
+// same rows (their columns 0..63 keep the block's values, which k_formq128
+// does not read: entries below a reflector's own index are zero by
+// definition), its d and e to entries 0..63 and 0..62 of the reversed
+// tridiagonal, as steps 64..125 of the whole reduction would write them;
+// e_127 = 0 is written here
+void launch_tridiag_lower128_tail(const ChunkArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL((k_tridiag_lower<64, true>), dim3(A.nzc), dim3(64), 0, st, A.z0, A.zorder,
+                     A.sel_ptr, A.ymat + 64 * 128, A.ymat + 64 * 128 + 64, A.dvec, A.evec, A.nzc,
+                     (int64_t)128 * 128, 128, 128);
+  hipLaunchKernelGGL(k_fill_elast128, dim3((A.nzc + 255) / 256), dim3(256), 0, st, A.evec, A.nzc);
 }
 
 
@@ -961,8 +1001,8 @@ void launch_chunk256(const ChunkArgs& A, hipStream_t st, PhaseTimer* tm) {
                        A.sel_ptr, A.sel_idx, A.sel_w, A.m, A.rho, A.innov, A.HS, A.mat, A.vec,
                        A.err);
   if (tm) tm->mark(PH_EIG);
-  hipLaunchKernelGGL(k_tridiag_lower<256>, dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder, A.sel_ptr,
-                     A.mat, A.ymat, A.dvec, A.evec, A.nzc);
+  hipLaunchKernelGGL((k_tridiag_lower<256>), dim3(A.nzc), dim3(512), 0, st, A.z0, A.zorder,
+                     A.sel_ptr, A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)256 * 256, 256, 256);
   launch_tridiag256_tail(A, st);
   hipLaunchKernelGGL(k_formq256, dim3(A.nzc, 4), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                      A.ymat, A.umat);

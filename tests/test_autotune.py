@@ -16,7 +16,7 @@ def test_space_leaves_out_the_wide_kernel_knobs_at_64():
     # every default is a candidate, and every candidate is a value the
     # library takes (oakg_set_tuning ranges)
     ok = {"slots": range(1, 5), "ql_zones": (4, 8, 16, 32), "graph": (0, 1), "order": (0, 1),
-          "tri_split": (0, 1, 2, 3), "dual": (0, 16, 32, 64), "persist": (0, 1, 2)}
+          "tri_split": (0, 1, 2, 3, 4), "dual": (0, 16, 32, 64), "persist": (0, 1, 2)}
     for k, vals in autotune.CANDIDATES.items():
         assert autotune.DEFAULT[k] in vals or k == "ql_zones"
         assert all(v in ok[k] for v in vals), k

@@ -155,12 +155,13 @@ def test_dual_excluded_observations_and_single_observation_zones():
     ctx.close()
 
 
-@pytest.mark.parametrize("split", [2, 3])
+@pytest.mark.parametrize("split", [2, 3, 4])
 @pytest.mark.parametrize("N", [100, 128])
 def test_three_launch_reduction_vs_oracle(N, split):
     """OAKG_TUNE_TRIDIAG_SPLIT = 2 (steps 32..63 of the Householder reduction
-    on 6 x 6 tiles) and 3 (steps 0..63 on the lower triangle only, the
-    upper part of A v from column sums): the partial sums are grouped
+    on 6 x 6 tiles), 3 (steps 0..63 on the lower triangle only, the upper
+    part of A v from column sums) and 4 (steps 64..125 too, by the NP = 64
+    kernel on the trailing block): the partial sums are grouped
     differently; against the oracle, with and without the observation-space
     bucket."""
     from oak_amd import assimilation as A

@@ -177,7 +177,7 @@ def test_tuning_rejects_bad_values():
     from oak_amd._lib import Context
     ctx = Context(0)
     for kw in (dict(slots=0), dict(slots=5), dict(ql_zones=12), dict(graph=2), dict(order=2),
-               dict(tri_split=4), dict(dual=48), dict(persist=3), dict(persist=-1)):
+               dict(tri_split=5), dict(dual=48), dict(persist=3), dict(persist=-1)):
         with pytest.raises(OakError):
             ctx.set_tuning(**kw)
     ctx.close()

@@ -909,6 +909,92 @@ __global__ __launch_bounds__(NP < 64 ? 64 : NP) void k_tql_apply(int64_t z0, con
 }
 
 // ---------------------------------------------------------------------------
+// U = Q Z for NP = 64 with the rows in registers: thread r holds row r of Q
+// whole (64 values), so a chain needs no other thread; it runs in unrolled,
+// branch-free segments of 16 positions with the identity (1, 0) outside the
+// chain, the (c, s) of each position from a table in LDS (the scheme of
+// k_tql_apply128's groups).  The operations on a row are those of
+// k_tql_apply.  One wave per zone.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_tql_apply64(int64_t z0, const int32_t* __restrict__ zorder,
+                                                     const int64_t* __restrict__ sel_ptr,
+                                                     double* __restrict__ qmat,
+                                                     const double2* __restrict__ rlog,
+                                                     int64_t cap,
+                                                     const int32_t* __restrict__ status) {
+  constexpr int NP = 64;
+  __shared__ double2 buf[2][NP + 12];
+  __shared__ double2 pr[NP];
+  const int zl = blockIdx.x;
+  const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;  // the zone is solved by the other pipeline
+  if (sel_ptr[z + 1] == sel_ptr[z]) return;
+  const int stz = status[zl];
+  if (stz & 3) return;  // failed zone: solved again by the fallback
+  const int tid = threadIdx.x;
+  double* Qr = qmat + (size_t)zl * NP * NP + (size_t)tid * NP;
+  double x[NP];
+  {
+    const double2* q2 = reinterpret_cast<const double2*>(Qr);
+#pragma unroll
+    for (int c = 0; c < NP / 2; ++c) {
+      const double2 v = q2[c];
+      x[2 * c] = v.x;
+      x[2 * c + 1] = v.y;
+    }
+  }
+  const int64_t len = stz >> 2;
+  const double2* lg = rlog + rlog_off(zl, cap);
+  if (tid < len) buf[0][tid] = lg[rlog_rec(tid)];
+  __syncthreads();
+  int64_t pos = 0;
+  int b = 0;
+  while (pos < len) {
+    const double2 hd = buf[b][0];
+    const int m = __builtin_amdgcn_readfirstlane(((int)hd.x) >> 16);
+    const int cnt = __builtin_amdgcn_readfirstlane((int)hd.y);
+    const int l = m - cnt;
+    const int64_t nxt = pos + 1 + cnt;
+    const bool pf = nxt + tid < len;
+    double2 pv = make_double2(0.0, 0.0);
+    if (pf) pv = lg[rlog_rec(nxt + tid)];
+    // positions m-1 .. l: rotation of columns P and P+1 with the record of
+    // position P
+    pr[tid] = (tid >= l && tid <= m - 1) ? buf[b][1 + (m - 1 - tid)] : make_double2(1.0, 0.0);
+    auto segment = [&](auto top) {
+      constexpr int T = decltype(top)::value;
+      if (m - 1 < T - 15 || l > T) return;
+      double2 qv[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (T - t >= 0) qv[t] = pr[T - t];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int J = T - t;
+        if (J >= 0) {
+          const double hh = x[J + 1], zj = x[J];
+          x[J + 1] = fma(qv[t].y, zj, qv[t].x * hh);
+          x[J] = fma(qv[t].x, zj, -qv[t].y * hh);
+        }
+      }
+    };
+    segment(std::integral_constant<int, 62>{});
+    segment(std::integral_constant<int, 46>{});
+    segment(std::integral_constant<int, 30>{});
+    segment(std::integral_constant<int, 14>{});
+    if (pf) buf[b ^ 1][tid] = pv;
+    __syncthreads();
+    pos = nxt;
+    b ^= 1;
+  }
+  {
+    double2* q2 = reinterpret_cast<double2*>(Qr);
+#pragma unroll
+    for (int c = 0; c < NP / 2; ++c) q2[c] = make_double2(x[2 * c], x[2 * c + 1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // U = Q Z for NP = 256 with the rows in registers, the scheme of
 // k_tql_apply128 (below) at twice the width: workgroup y takes rows
 // 128 y .. 128 y + 127, row r's columns 0..127 in thread r (waves 0-1, the
@@ -1585,8 +1671,13 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
                        A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status, A.dvec, A.nzc,
                        A.vec, A.ampl, A.err, A.prec == 1 ? 0 : 1);
   } else {
-    hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0, A.zorder,
-                       A.sel_ptr, A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status);
+    if constexpr (NP == 64)
+      hipLaunchKernelGGL(k_tql_apply64, dim3(A.nzc), dim3(64), 0, st, A.z0, A.zorder, A.sel_ptr,
+                         A.ymat, (const double2*)A.rlog, A.rlog_cap, A.tql_status);
+    else
+      hipLaunchKernelGGL(k_tql_apply<NP>, dim3(A.nzc), dim3(NP < 64 ? 64 : NP), 0, st, A.z0,
+                         A.zorder, A.sel_ptr, A.ymat, (const double2*)A.rlog, A.rlog_cap,
+                         A.tql_status);
     hipLaunchKernelGGL(k_eigvec_epi<NP>, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.N, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.nzc, A.vec, A.ampl, A.err);
   }

@@ -3,7 +3,7 @@ Gram on 64 x 64 blocks, the Householder reduction on 8 x 8 register tiles of
 the lower triangle, Q in four column blocks, QL with the rotations on 64-row
 blocks, the transform and update in 64-row / 64-column blocks.  Against the
 oracle at the FP64 bound (1e-10 of the largest increment / anomaly) and the
-mixed bound (1e-4), on column-ocean workloads whose zones have from 0 to a
+mixed bound (1e-4, at N = 200 and 256), on column-ocean workloads whose zones have from 0 to a
 few hundred observations (0 to about 500) (so p < N, p ~ N and p > N all occur), with
 inflation and a binding clamp; locAnalysis on a CSR selection with zones of
 1 to 300 elements; the same bits under the zone order and graph replay.
@@ -73,7 +73,7 @@ def test_wide_fp64_vs_oracle(N):
     ctx.close()
 
 
-@pytest.mark.parametrize("N", [256])
+@pytest.mark.parametrize("N", [200, 256])
 def test_wide_mixed_vs_oracle(N):
     from oak_amd import assimilation as A
     P = _problem(N, 1000 + N)

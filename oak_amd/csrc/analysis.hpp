@@ -178,6 +178,8 @@ void launch_tridiag_lower128(const ChunkArgs& A, hipStream_t st);
 // OAKG_TUNE_TRIDIAG_SPLIT 4: steps 64..125 by the lower-triangle kernel at
 // NP = 64 on the trailing block (after launch_tridiag_lower128)
 void launch_tridiag_lower128_tail(const ChunkArgs& A, hipStream_t st);
+// NP = 64: reduction on the lower triangle and Q on MFMA (wide256.hip)
+void launch_tridiag_lower64(const ChunkArgs& A, hipStream_t st);
 void launch_prep_ens_f32(int NP, const PrepEnsArgs& A, hipStream_t st);
 void launch_update_f32(int NP, const ChunkArgs& A, hipStream_t st);
 void launch_nll_adjoint(int NP, const ChunkArgs& A, hipStream_t st);

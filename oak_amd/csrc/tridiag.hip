@@ -1636,6 +1636,9 @@ static void tql_pipeline_t(const ChunkArgs& A, hipStream_t st) {
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
     hipLaunchKernelGGL((k_tridiag<128, 2>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);
+  } else if (NP == 64) {
+    // the lower-triangle reduction on one wave per zone, Q on MFMA
+    launch_tridiag_lower64(A, st);
   } else {
     hipLaunchKernelGGL((k_tridiag<NP, 0>), dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr,
                        A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)NP * NP, 1);

@@ -27,7 +27,9 @@
 // post-processing of assimilation.F90:3299-3363); only the split of the work
 // and the order of the partial sums differ.  k_tridiag_lower<128> (the same
 // lower-triangle reduction at NP = 128, steps 0..63) also serves the
-// N = 128 path as its reduction option 3 (launch_tridiag_lower128).
+// N = 128 path as its reduction options 3 and 4 (launch_tridiag_lower128),
+// k_tridiag_lower<64, true> the trailing block of option 4 and the whole
+// reduction at NP = 64 (with k_formq64 for Q).
 #include "analysis.hpp"
 #include "common.hpp"
 #include "eigen_common.hpp"
@@ -755,6 +757,144 @@ __global__ __launch_bounds__(256) void k_formq256(int64_t z0, const int32_t* __r
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       U[(size_t)(16 * I + lr + 4 * r) * NP + (NP - 1 - (16 * J + lc))] = q[I][r];
+}
+
+// ---------------------------------------------------------------------------
+// Q = H_0 ... H_61 for NP = 64 (the reflectors k_tridiag_lower<64, true>
+// leaves in the rows of qmat), the scheme of k_formq256 with one workgroup
+// per zone: wave w holds tile column w of Q (4 tiles), blocks of 16
+// reflectors in compact WY form on MFMA.  Every reflector is read before Q
+// is written over them (in place, columns reversed as the QL kernels expect).
+// ---------------------------------------------------------------------------
+struct FormQ64Lds {
+  static constexpr int LV = 17;
+  double vb[2][64 * LV];
+  double tb[2][16 * LV];
+  double g[16 * LV];
+  double tau[2][16];
+};
+
+__global__ __launch_bounds__(256) void k_formq64(int64_t z0, const int32_t* __restrict__ zorder,
+                                                 const int64_t* __restrict__ sel_ptr,
+                                                 double* __restrict__ qmat) {
+  constexpr int NP = 64, NB = 4, LV = FormQ64Lds::LV;
+  __shared__ FormQ64Lds S;
+  const int zl = blockIdx.x;
+  const int64_t z = zone_at(zorder, z0 + zl);
+  if (z < 0) return;
+  if (sel_ptr[z + 1] == sel_ptr[z]) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int lr = lane >> 4, lc = lane & 15;
+  double* V = qmat + (size_t)zl * NP * NP;
+  const int J = wave;
+  d4 q[NB];
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[I][r] = (I == J && lr + 4 * r == lc) ? 1.0 : 0.0;
+  // thread t moves reflector pj = t / 16 of a block, rows pr + 16 e (e < 4)
+  const int pj = t >> 4, pr = t & 15;
+  double pf[4];
+  auto load_block = [&](int b) {
+    const int k = 16 * b + pj;
+    if (k < NP - 2) {
+      const double* src = V + (size_t)k * NP + pr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pf[e] = src[16 * e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pf[e] = 0.0;
+    }
+  };
+  load_block(NB - 1);
+  for (int b = NB - 1; b >= 0; --b) {
+    const int buf = b & 1;
+    double* vb = S.vb[buf];
+    double* tb = S.tb[buf];
+    {
+      const int k = 16 * b + pj;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = pr + 16 * e;
+        vb[r * LV + pj] = r <= k ? 0.0 : pf[e];  // v_k is zero above row k + 1
+        if (r == k) S.tau[buf][pj] = pf[e];
+      }
+    }
+    if (b > 0) load_block(b - 1);
+    lds_barrier();
+    // wave 0: G = Vb^T Vb (rows >= 16 b), then Tb column by column:
+    // Tb[j][j] = tau_j, Tb[0:j, j] = -tau_j Tb[0:j, 0:j] G[0:j, j]
+    if (wave == 0) {
+      d4 g = {0.0, 0.0, 0.0, 0.0};
+      for (int kk = 4 * b; kk < 16; ++kk) {
+        const double xv = vb[(4 * kk + lr) * LV + lc];
+        g = mfma_f64(xv, xv, g);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) S.g[(lr + 4 * r) * LV + lc] = g[r];
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 16) {
+        double trr[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          asm volatile("" ::: "memory");  // G column by column, not all loads at once
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < j; ++k) s = fma(trr[k], S.g[k * LV + j], s);
+          const double tj = S.tau[buf][j];
+          trr[j] = lane < j ? -tj * s : (lane == j ? tj : 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) tb[lane * LV + j] = trr[j];
+      }
+    }
+    // W = Vb^T Q[:, J]
+    d4 w = {0.0, 0.0, 0.0, 0.0};
+    if (J >= b) {
+#pragma unroll
+      for (int I = 0; I < NB; ++I)
+        if (I >= b) {
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+            w = mfma_f64(vb[(16 * I + 4 * kk + lr) * LV + lc], q[I][kk], w);
+        }
+    }
+    lds_barrier();  // Tb
+    // W' = -Tb W;  Q[I][J] += Vb[I rows] W'
+    if (J >= b) {
+      d4 wp = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) wp = mfma_f64(-tb[lc * LV + 4 * kk + lr], w[kk], wp);
+#pragma unroll
+      for (int I = 0; I < NB; ++I)
+        if (I >= b) {
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+            q[I] = mfma_f64(vb[(16 * I + lc) * LV + 4 * kk + lr], wp[kk], q[I]);
+        }
+    }
+  }
+  // every reflector row was read before the last barrier
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      V[(size_t)(16 * I + lr + 4 * r) * NP + (NP - 1 - (16 * J + lc))] = q[I][r];
+}
+
+// NP = 64 (33 <= N <= 64, and the observation-space zones of N = 128): the
+// reduction on the lower triangle in full (one wave per zone), e_63 = 0,
+// then Q by k_formq64
+__global__ void k_fill_elast64(double* __restrict__ evec, int nzc) {
+  const int zl = blockIdx.x * 256 + threadIdx.x;
+  if (zl < nzc) evec[(size_t)63 * nzc + zl] = 0.0;
+}
+
+void launch_tridiag_lower64(const ChunkArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL((k_tridiag_lower<64, true>), dim3(A.nzc), dim3(64), 0, st, A.z0, A.zorder,
+                     A.sel_ptr, A.mat, A.ymat, A.dvec, A.evec, A.nzc, (int64_t)64 * 64, 64, 64);
+  hipLaunchKernelGGL(k_fill_elast64, dim3((A.nzc + 255) / 256), dim3(256), 0, st, A.evec, A.nzc);
+  hipLaunchKernelGGL(k_formq64, dim3(A.nzc), dim3(256), 0, st, A.z0, A.zorder, A.sel_ptr, A.ymat);
 }
 
 // ---------------------------------------------------------------------------

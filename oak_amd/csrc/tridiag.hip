@@ -13,16 +13,21 @@
 //                  the sequential recurrences of many zones run side by side.
 //                  Every plane rotation is logged; d ends up holding the
 //                  eigenvalues.
-//   k_tql_apply    (N <= 64) one workgroup per zone: applies the logged
-//                  rotations to the rows of Q in LDS, giving U = Q Z.
-//   k_eigvec_epi   (N <= 64) per zone: a = U diag(1/mu) U^T b (+ one
+//   k_tql_apply    (N <= 32) one workgroup per zone: applies the logged
+//                  rotations to the rows of Q in LDS, giving U = Q Z;
+//                  k_tql_apply64 (33 <= N <= 64) the same on rows held whole
+//                  in registers, k_tql_apply256 (N > 128) on 128-row blocks,
+//                  each row split over two threads.
+//   k_eigvec_epi   (N <= 64, N > 128) per zone: a = U diag(1/mu) U^T b (+ one
 //                  refinement step against M), v ~ U diag(mu^1/2) U^T 1,
 //                  Y = U diag(mu^-1/4), with mu = max(lambda, 1)
 //                  (rrsqrt.F90:137-142, 176-177); mu itself is kept beside a
 //                  and v for the likelihood.
-//   k_tql_apply128 (N > 64) the rotations on rows held in registers (each
-//                  row split over two threads), then the epilogue of
+//   k_tql_apply128 (65 <= N <= 128) the rotations on rows held in registers
+//                  (each row split over two threads), then the epilogue of
 //                  k_eigvec_epi.
+// The reductions on the lower triangle only (k_tridiag_lower, with Q formed
+// by k_formq64 at NP = 64) are in wide256.hip.
 #include <type_traits>
 
 #include "analysis.hpp"

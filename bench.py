@@ -362,8 +362,8 @@ def roofline(F, phase, mixed, eig_name, config, N):
 # the kernels of each phase at N > 64 (the observation-space bucket's
 # kernels and the Jacobi fallback passes included)
 PHASE_KERNELS = {"eig": ["k_tridiag", "k_tridiag_lower", "k_formq128", "k_tql", "k_tql_apply128",
-                         "k_eig128", "k_tql_apply", "k_tql_apply64", "k_eigvec_epi", "k_eig",
-                         "k_dual_lift"],
+                         "k_eig128", "k_tql_apply", "k_tql_apply64", "k_formq64", "k_eigvec_epi",
+                         "k_eig", "k_dual_lift"],
                  "gram": ["k_gram128", "k_gram128_persist", "k_dual_gram"],
                  "update": ["k_xform128", "k_xform128f"]}
 
@@ -520,9 +520,11 @@ def main():
                  "k_tridiag_lower<128>+k_tridiag_lower<64>+k_formq128+k_tql+k_tql_apply128 "
                  "(Householder on the lower triangle, Q in "
                  "blocked WY form on FP64 MFMA, implicit QL, eigenvector epilogue; FP64), with "
-                 "the zones of p < 64 through k_tridiag+k_tql+k_tql_apply64+k_eigvec_epi at 64 "
+                 "the zones of p < 64 through k_tridiag_lower<64>+k_formq64+k_tql+k_tql_apply64+"
+                 "k_eigvec_epi at 64 "
                  "and k_dual_lift" if N > 64 else
-                 "k_tridiag+k_tql+k_tql_apply(64)+k_eigvec_epi (Householder + implicit QL, FP64)")
+                 "k_tridiag_lower<64>+k_formq64+k_tql+k_tql_apply64+k_eigvec_epi (Householder on "
+                 "the lower triangle + implicit QL, FP64; k_tridiag below N = 33)")
                 if args.eig == "ql" else "k_eig (one-sided Jacobi, FP64)")
     eig_id = OAKG_EIG_QL if args.eig == "ql" else OAKG_EIG_JACOBI
     res = {}

@@ -1,7 +1,8 @@
 """Ensembles of 129..256 members (NP = 256, oak_amd/csrc/wide256.hip): the
-Gram on 64 x 64 blocks, the Householder reduction on 8 x 8 register tiles of
-the lower triangle, Q in four column blocks, QL with the rotations on 64-row
-blocks, the transform and update in 64-row / 64-column blocks.  Against the
+Gram on 16 waves of 32 x 32 blocks, the Householder reduction on 8 x 8
+register tiles of the lower triangle, Q in four column blocks, QL with each
+row's rotations in registers, the transform in 64-row blocks and the update
+in 32-column blocks.  Against the
 oracle at the FP64 bound (1e-10 of the largest increment / anomaly) and the
 mixed bound (1e-4, at N = 200 and 256), on column-ocean workloads whose zones have from 0 to a
 few hundred observations (0 to about 500) (so p < N, p ~ N and p > N all occur), with
